@@ -1,0 +1,167 @@
+"""FMM apply throughput on one MI355X: elastostatic single+double layer on
+flat triangles (elfmm3dtriatarg), workload C2 of the baseline:
+  icosphere 64x64 per face (81,920 triangles), densities uniform in [-1,1]^3,
+  evaluation at all centroids plus 100,000 targets uniform in [-2,2]^3 minus
+  the unit ball, lam = mu = 1, FP64, iprec 2 (tolerance 0.5e-6).
+A step is one complete apply (tree + far field + near field + assembly) with
+inputs already resident in HBM. Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--iprec 2] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64 = 78.6e12        # MI355X FP64 vector / matrix dense peak, FLOP/s
+PEAK_HBM = 8.0e12          # bytes/s
+
+
+def workload(nsplit=64, ntarg=100000, seed=2024):
+    from elfmmlib3d_amd import mesh
+    rng = np.random.default_rng(seed)
+    tri = mesh.icosphere(nsplit)
+    nrm = mesh.triangle_normals(tri)
+    src = mesh.centroids(tri)
+    ssl = rng.uniform(-1, 1, (tri.shape[0], 3))
+    sdl = rng.uniform(-1, 1, (tri.shape[0], 3))
+    tg = mesh.cube_targets(ntarg, 2.0, 1.0, rng)
+    return tri, nrm, src, ssl, sdl, tg
+
+
+def cpu_baseline(seconds_budget=30.0):
+    """The reference's own FMM (oracle/_ref, OpenMP) on a bounded sample of
+    the same workload: icosphere 16x16 per face (5,120 triangles) with 6,250
+    targets in the same cube (same target/triangle ratio as C2), iprec 2."""
+    from oracle import ref
+    if not ref.available():
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    tri, nrm, src, ssl, sdl, tg = workload(16, 6250, 7)
+    pts = tri.shape[0] + tg.shape[0]
+    times = []
+    t_end = time.time() + seconds_budget
+    while True:
+        t0 = time.time()
+        ier, *_ = ref.elfmm3dtriatarg(2, 1.0, 1.0, tri, nrm, src, ssl, sdl, tg)
+        times.append(time.time() - t0)
+        if time.time() > t_end or len(times) >= 3:
+            break
+    t = float(np.median(times))
+    return {"value": pts / t / 1e6, "unit": "M tris+targets/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"reference elfmm3dtriatarg (flang -O1 -fopenmp), icosphere 16x16/face "
+                      f"(5120 tris) + 6250 cube targets, iprec 2, median of {len(times)} applies "
+                      f"({t:.2f} s each)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--iprec", type=int, default=2)
+    ap.add_argument("--nsplit", type=int, default=64)
+    ap.add_argument("--ntarg", type=int, default=100000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--check", type=int, default=2000, help="targets checked against the direct sum")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+    import elfmmlib3d_amd as E
+
+    dev = torch.device("cuda:0")
+    tri, nrm, src, ssl, sdl, tg = workload(args.nsplit, args.ntarg)
+    n, m = tri.shape[0], tg.shape[0]
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    d = dict(triangle=T(tri), trinorm=T(nrm), source=T(src), sigma_sl=T(ssl), sigma_dl=T(sdl), target=T(tg))
+    out = dict(ptfrc=torch.empty((n, 3), dtype=torch.float64, device=dev),
+               strain=torch.empty((n, 3, 3), dtype=torch.float64, device=dev),
+               ptfrctarg=torch.empty((m, 3), dtype=torch.float64, device=dev),
+               straintarg=torch.empty((m, 3, 3), dtype=torch.float64, device=dev))
+
+    def step():
+        ier = E.elfmm3dtriatarg_dev(args.iprec, 1.0, 1.0, d["triangle"], d["trinorm"], d["source"],
+                                    d["sigma_sl"], d["sigma_dl"], d["target"], out["ptfrc"], out["strain"],
+                                    out["ptfrctarg"], out["straintarg"])
+        if ier != 0:
+            raise RuntimeError(f"ier={ier}")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lat = []
+    phases = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        step()
+        lat.append(time.perf_counter() - ts)
+        phases.append(E.last_timings())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stats = E.last_stats()
+    total = t1 - t0
+    ms_per_step = 1e3 * total / args.steps
+    value = (n + m) * args.steps / total / 1e6
+    ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+
+    # accuracy against the direct sum at sampled targets
+    err = {}
+    if args.check:
+        rng = np.random.default_rng(5)
+        idx = rng.choice(m, size=min(args.check, m), replace=False)
+        dp, ds_, dpt, dst = E.el3dtriadirecttarg(1.0, 1.0, tri, nrm, src, ssl, sdl, tg[idx], 0, 0, 1, 1)
+        fp = out["ptfrctarg"].cpu().numpy()[idx]
+        fs = out["straintarg"].cpu().numpy()[idx]
+        err = {"disp_rel_l2": float(np.linalg.norm(fp - dpt) / np.linalg.norm(dpt)),
+               "strain_rel_l2": float(np.linalg.norm(fs - dst) / np.linalg.norm(dst)),
+               "checked_targets": int(len(idx))}
+
+    # dominant kernel and its roofline
+    kernels = {k: ph[k] for k in ("p2m", "m2m", "m2l", "p2l", "l2l", "l2t", "m2t", "near")}
+    dom = max(kernels, key=kernels.get)
+    roof = None
+    if dom == "m2l":
+        t = ph["m2l"] / 1e3
+        alg = stats["m2l_flops_alg"]
+        roof = {"kernel": "k_gemm (M2L)", "bound": "mfma", "achieved": alg / t / 1e12,
+                "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": alg / t / PEAK_FP64,
+                "traffic": None, "executed_tflops": stats["m2l_flops_dense"] / t / 1e12,
+                "algorithmic_flop_per_launch": alg, "ms_per_launch": ph["m2l"]}
+    else:
+        roof = {"kernel": dom, "bound": "mfma", "achieved": None, "peak": PEAK_FP64 / 1e12,
+                "unit": "TFLOP/s", "frac": None, "traffic": None, "ms_per_launch": kernels[dom]}
+
+    cpu = None if args.no_cpu else cpu_baseline()
+    lat_ms = np.array(lat) * 1e3
+    line = {
+        "metric": "FMM apply throughput (M tris+targets/s) @1e-6 err; p50/p99 served latency",
+        "value": value, "unit": "M tris+targets/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "C2: icosphere 64x64/face (81,920 tris) SL+DL, centroids + 100k cube "
+                               "targets, lam=mu=1", "iprec": args.iprec, "triangles": n, "targets": m,
+                   "tolerance": {1: 0.5e-3, 2: 0.5e-6, 3: 0.5e-9}.get(args.iprec)},
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+        "accuracy_vs_direct": err,
+        "phases_ms": ph, "work": stats,
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["speedup_vs_cpu_reference"] = value / cpu["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,149 @@
+"""ctypes bindings of libelfmm_gfx950.so (see include/elfmm_gfx950.h)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from ._build import LIB
+
+_lib = None
+
+
+def library_path():
+    return LIB
+
+
+def library():
+    """Load the in-tree library; raises if it has not been built.
+
+    PyTorch-ROCm ships its own copy of the HIP runtime (same soname). Loading
+    torch first makes this library bind to that already-loaded runtime, so
+    one process never holds two HIP runtimes (the second one would find no
+    GPU) and torch tensors can be passed to the device entry points."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} is not built: run elfmmlib3d_amd.build()")
+        try:
+            import torch  # noqa: F401  (loads torch's HIP runtime first)
+        except ImportError:
+            pass
+        _lib = C.CDLL(LIB)
+        _lib.elfmm_gfx950_triatarg_dev.restype = C.c_int
+        _lib.elfmm_gfx950_version.restype = C.c_char_p
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, shape):
+    return np.ascontiguousarray(a, dtype=np.float64).reshape(shape)
+
+
+def _iref(v):
+    return C.byref(C.c_int(int(v)))
+
+
+def _dref(v):
+    return C.byref(C.c_double(float(v)))
+
+
+def _prep(triangle, trinorm, source, sigma_sl, sigma_dl, target):
+    n = triangle.shape[0]
+    tri = _f64(triangle, (n, 3, 3))
+    nrm = _f64(trinorm, (n, 3))
+    src = _f64(source, (n, 3))
+    ifs = int(sigma_sl is not None)
+    ifd = int(sigma_dl is not None)
+    ssl = _f64(sigma_sl if ifs else np.zeros((n, 3)), (n, 3))
+    sdl = _f64(sigma_dl if ifd else np.zeros((n, 3)), (n, 3))
+    if target is None:
+        target = np.zeros((0, 3))
+    m = target.shape[0]
+    tgt = _f64(target if m else np.zeros((1, 3)), (max(m, 1), 3))
+    return n, tri, nrm, src, ifs, ssl, ifd, sdl, m, tgt
+
+
+def elfmm3dtriatarg(iprec, rlam, rmu, triangle, trinorm, source, sigma_sl=None, sigma_dl=None,
+                    target=None, ifptfrc=1, ifstrain=1, ifptfrctarg=1, ifstraintarg=1):
+    """FMM displacement and strain at centroids and targets
+    (src/elfmm3dtria.f:162). Returns (ier, ptfrc, strain, ptfrctarg, straintarg)."""
+    n, tri, nrm, src, ifs, ssl, ifd, sdl, m, tgt = _prep(triangle, trinorm, source, sigma_sl,
+                                                          sigma_dl, target)
+    pt, st = np.zeros((n, 3)), np.zeros((n, 3, 3))
+    ptt, stt = np.zeros((max(m, 1), 3)), np.zeros((max(m, 1), 3, 3))
+    ier = C.c_int(0)
+    library().elfmm3dtriatarg_(C.byref(ier), _iref(iprec), _dref(rlam), _dref(rmu), _p(tri), _p(nrm),
+                               _iref(n), _p(src), _iref(ifs), _p(ssl), _iref(ifd), _p(sdl),
+                               _iref(ifptfrc), _p(pt), _iref(ifstrain), _p(st), _iref(m), _p(tgt),
+                               _iref(ifptfrctarg if m else 0), _p(ptt),
+                               _iref(ifstraintarg if m else 0), _p(stt))
+    return ier.value, pt, st, ptt[:m], stt[:m]
+
+
+def elfmm3dtria(iprec, rlam, rmu, triangle, trinorm, source, sigma_sl=None, sigma_dl=None,
+                ifptfrc=1, ifstrain=1):
+    """On-surface FMM (src/elfmm3dtria.f:71). Returns (ier, ptfrc, strain)."""
+    n, tri, nrm, src, ifs, ssl, ifd, sdl, _, _ = _prep(triangle, trinorm, source, sigma_sl,
+                                                        sigma_dl, None)
+    pt, st = np.zeros((n, 3)), np.zeros((n, 3, 3))
+    ier = C.c_int(0)
+    library().elfmm3dtria_(C.byref(ier), _iref(iprec), _dref(rlam), _dref(rmu), _p(tri), _p(nrm),
+                           _iref(n), _p(src), _iref(ifs), _p(ssl), _iref(ifd), _p(sdl),
+                           _iref(ifptfrc), _p(pt), _iref(ifstrain), _p(st))
+    return ier.value, pt, st
+
+
+def el3dtriadirecttarg(rlam, rmu, triangle, trinorm, source, sigma_sl=None, sigma_dl=None,
+                       target=None, ifptfrc=1, ifstrain=1, ifptfrctarg=1, ifstraintarg=1):
+    """O(N^2) direct sum on the GPU (src/elfmm3dtria.f:1453).
+    Returns (ptfrc, strain, ptfrctarg, straintarg)."""
+    n, tri, nrm, src, ifs, ssl, ifd, sdl, m, tgt = _prep(triangle, trinorm, source, sigma_sl,
+                                                          sigma_dl, target)
+    pt, st = np.zeros((n, 3)), np.zeros((n, 3, 3))
+    ptt, stt = np.zeros((max(m, 1), 3)), np.zeros((max(m, 1), 3, 3))
+    library().el3dtriadirecttarg_(_dref(rlam), _dref(rmu), _p(tri), _p(nrm), _iref(n), _p(src),
+                                  _iref(ifs), _p(ssl), _iref(ifd), _p(sdl), _iref(ifptfrc), _p(pt),
+                                  _iref(ifstrain), _p(st), _iref(m), _p(tgt),
+                                  _iref(ifptfrctarg if m else 0), _p(ptt),
+                                  _iref(ifstraintarg if m else 0), _p(stt))
+    return pt, st, ptt[:m], stt[:m]
+
+
+def elfmm3dtriatarg_dev(iprec, rlam, rmu, triangle, trinorm, source, sigma_sl, sigma_dl, target,
+                        ptfrc, strain, ptfrctarg, straintarg, stream=None):
+    """Device-resident apply on torch CUDA/HIP tensors (float64, contiguous).
+    Any of sigma_sl/sigma_dl/target/outputs may be None. Returns ier."""
+    def ptr(t):
+        return C.c_void_p(0 if t is None else t.data_ptr())
+    n = triangle.shape[0]
+    m = 0 if target is None else target.shape[0]
+    s = C.c_void_p(0 if stream is None else stream.cuda_stream)
+    return library().elfmm_gfx950_triatarg_dev(
+        C.c_int(iprec), C.c_double(rlam), C.c_double(rmu), C.c_int(n), ptr(triangle), ptr(trinorm),
+        ptr(source), C.c_int(sigma_sl is not None), ptr(sigma_sl), C.c_int(sigma_dl is not None),
+        ptr(sigma_dl), C.c_int(ptfrc is not None), ptr(ptfrc), C.c_int(strain is not None),
+        ptr(strain), C.c_int(m), ptr(target), C.c_int(ptfrctarg is not None), ptr(ptfrctarg),
+        C.c_int(straintarg is not None), ptr(straintarg), s)
+
+
+def last_stats():
+    """Work counts of the last apply on this thread (see include/elfmm_gfx950.h)."""
+    out = np.zeros(16)
+    library().elfmm_gfx950_last_stats(_p(out))
+    keys = ["nboxes", "nlev", "p", "m2l_pairs", "m2l_flops_dense", "m2l_flops_alg", "near_pairs",
+            "m2l_tiles", "m2m_pairs", "l2l_pairs", "p2m_boxes", "p2l_pairs", "m2t_pairs", "l2t_boxes"]
+    return dict(zip(keys, out.tolist()))
+
+
+def last_timings():
+    """Per-phase times (ms) of the last apply on this thread (ELFMM_TIMING=1)."""
+    out = np.zeros(12)
+    library().elfmm_gfx950_last_timings(_p(out))
+    keys = ["plan", "upload", "p2m", "m2m", "m2l", "p2l", "l2l", "l2t", "m2t", "near",
+            "assemble", "total"]
+    return dict(zip(keys, out.tolist()))

@@ -1,0 +1,94 @@
+/*
+ * elfmm_gfx950: MI355X (gfx950) elastostatic triangle layer-potential FMM.
+ *
+ * C-ABI of libelfmm_gfx950.so. Two families of entry points:
+ *
+ *  1. Drop-in replacements of the reference's Fortran subroutines, with the
+ *     reference's symbol convention (lowercase, trailing underscore, every
+ *     argument by reference, Fortran column-major arrays, host memory). A
+ *     Fortran program that linked the reference's objects links this
+ *     library instead; iso_c_binding interfaces are in
+ *     elfmmlib3d_amd/fortran/elfmm_gfx950.f90.
+ *
+ *  2. Device-resident entry points (same argument meaning, device pointers
+ *     and a HIP stream) for callers that keep their data in HBM.
+ *
+ * Array layouts (Fortran): triangle(3,3,n) = vertex coordinates, trinorm(3,n)
+ * unit normals, source(3,n) centroids, sigma_sl/sigma_dl(3,n) densities,
+ * target(3,nt), ptfrc(3,*) displacement, strain(3,3,*). Outputs are not
+ * scaled by 1/(4 pi), as in the reference.
+ *
+ * Error codes (ier): 0 ok; 4 tree construction failed; 8 device memory
+ * allocation failed; 16 unsupported precision flag on this device path
+ * (iprec > 3); 32 HIP runtime error. Errors never abort the process.
+ */
+#ifndef ELFMM_GFX950_H
+#define ELFMM_GFX950_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Replaces elfmm3dtria (src/elfmm3dtria.f:71-74): evaluation at the
+ * triangle centroids only. */
+void elfmm3dtria_(int *ier, const int *iprec, const double *rlam, const double *rmu,
+                  const double *triangle, const double *trinorm, const int *nparts,
+                  const double *source, const int *ifsingle, const double *sigma_sl,
+                  const int *ifdouble, const double *sigma_dl, const int *ifptfrc,
+                  double *ptfrc, const int *ifstrain, double *strain);
+
+/* Replaces elfmm3dtriatarg (src/elfmm3dtria.f:162-167). */
+void elfmm3dtriatarg_(int *ier, const int *iprec, const double *rlam, const double *rmu,
+                      const double *triangle, const double *trinorm, const int *nparts,
+                      const double *source, const int *ifsingle, const double *sigma_sl,
+                      const int *ifdouble, const double *sigma_dl, const int *ifptfrc,
+                      double *ptfrc, const int *ifstrain, double *strain, const int *ntargs,
+                      const double *target, const int *ifptfrctarg, double *ptfrctarg,
+                      const int *ifstraintarg, double *straintarg);
+
+/* Replaces el3dtriadirecttarg (src/elfmm3dtria.f:1453-1458): the O(N^2)
+ * direct sum, evaluated on the GPU with the same pair quadratures. */
+void el3dtriadirecttarg_(const double *rlam, const double *rmu, const double *triangle,
+                         const double *trinorm, const int *nsource, const double *source,
+                         const int *ifsingle, const double *sigma_sl, const int *ifdouble,
+                         const double *sigma_dl, const int *ifptfrc, double *ptfrc,
+                         const int *ifstrain, double *strain, const int *ntarget,
+                         const double *target, const int *ifptfrctarg, double *ptfrctarg,
+                         const int *ifstraintarg, double *straintarg);
+
+/* Device-resident elfmm3dtriatarg: all arrays are device pointers with the
+ * Fortran layouts above; the call is enqueued on `stream` (a hipStream_t,
+ * NULL for the default stream) and returns after the outputs are written.
+ * No counterpart in the reference: it is the same operation for callers
+ * whose data already lives in HBM. */
+int elfmm_gfx950_triatarg_dev(int iprec, double rlam, double rmu, int nparts,
+                              const double *d_triangle, const double *d_trinorm,
+                              const double *d_source, int ifsingle, const double *d_sigma_sl,
+                              int ifdouble, const double *d_sigma_dl, int ifptfrc,
+                              double *d_ptfrc, int ifstrain, double *d_strain, int ntargs,
+                              const double *d_target, int ifptfrctarg, double *d_ptfrctarg,
+                              int ifstraintarg, double *d_straintarg, void *stream);
+
+/* Per-phase times (milliseconds) of the last call on this thread:
+ * [0] tree+plan (host), [1] upload, [2] P2M, [3] M2M, [4] M2L, [5] P2L,
+ * [6] L2L, [7] L2T, [8] M2T, [9] near field, [10] assembly, [11] total
+ * wall time of the call. [0] is host time; [1]-[10] are HIP-event times on
+ * the stream each phase runs on (the near field overlaps [2]-[8]). */
+void elfmm_gfx950_last_timings(double *ms12);
+
+/* Work counts of the last call on this thread: [0] boxes, [1] levels,
+ * [2] expansion order p, [3] M2L pairs, [4] M2L flops as executed (dense
+ * operators, four right-hand sides), [5] M2L flops of the reference's
+ * rotate/shift/rotate scheme for the same pairs, [6] near-field (point,
+ * triangle) pairs, [7] M2L tiles, [8] M2M pairs, [9] L2L pairs, [10] P2M
+ * boxes, [11] P2L (box, source box) pairs, [12] M2T (box, source box) pairs,
+ * [13] L2T boxes, [14]-[15] reserved. */
+void elfmm_gfx950_last_stats(double *st16);
+
+/* Library version string. */
+const char *elfmm_gfx950_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,151 @@
+"""Write the golden fixtures from the reference compiled from its sources.
+
+Run in a container that has the reference tree (make -C oracle ref first):
+    python tests/golden/make_golden.py
+Each fixture holds its inputs and the reference's outputs, so the tests
+never need the reference itself. Cases (all FP64):
+  fmm_s6        icosphere 6x6/face (720 tris) + 300 shell targets, iprec 2,
+                lam=1.3 mu=0.9, single+double layer, all outputs
+  fmm_c1        config C1: icosphere 12x12/face (2880 tris) + 1000 targets in
+                1.5<r<2, lam=mu=1, iprec 2
+  fmm_c1_p1     C1 geometry at iprec 1
+  fmm_sl_only   fmm_s6 geometry, single layer only, targets only outputs
+  fmm_dl_only   fmm_s6 geometry, double layer only, centroid outputs only
+  fmm_tiny      one-level tree (40 tris, all near field)
+  pairs         500 random (target, triangle) pairs incl. self and
+                near-coplanar targets: elust3/eltst3triadirecttarg_one
+  tables        random plane-triangle tables (triabtable / triartable)
+  tree_c1       boxes, permutations and lists of d3tstrcr on C1 points
+  direct_s4     el3dtriadirecttarg on a 320-triangle sphere + 50 targets
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from elfmmlib3d_amd import mesh  # noqa: E402
+from oracle import ref  # noqa: E402
+
+
+def sphere_case(nsplit, ntarg, seed, rmin=1.5, rmax=2.0):
+    rng = np.random.default_rng(seed)
+    tri = mesh.icosphere(nsplit)
+    nrm = mesh.triangle_normals(tri)
+    src = mesh.centroids(tri)
+    ssl = rng.uniform(-1, 1, (tri.shape[0], 3))
+    sdl = rng.uniform(-1, 1, (tri.shape[0], 3))
+    tg = mesh.shell_targets(ntarg, rmin, rmax, rng)
+    return tri, nrm, src, ssl, sdl, tg
+
+
+def save(name, **kw):
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **kw)
+    print("wrote", path, os.path.getsize(path))
+
+
+def fmm_case(name, iprec, lam, mu, tri, nrm, src, ssl, sdl, tg, flags=(1, 1, 1, 1)):
+    ier, pt, st, ptt, stt = ref.elfmm3dtriatarg(iprec, lam, mu, tri, nrm, src, ssl, sdl, tg, *flags)
+    assert ier == 0
+    save(name, iprec=iprec, lam=lam, mu=mu, triangle=tri, trinorm=nrm, source=src,
+         sigma_sl=ssl if ssl is not None else np.zeros(0), sigma_dl=sdl if sdl is not None else np.zeros(0),
+         target=tg, flags=np.array(flags), ptfrc=pt, strain=st, ptfrctarg=ptt, straintarg=stt)
+
+
+def main():
+    tri, nrm, src, ssl, sdl, tg = sphere_case(6, 300, 11)
+    fmm_case("fmm_s6", 2, 1.3, 0.9, tri, nrm, src, ssl, sdl, tg)
+    fmm_case("fmm_sl_only", 2, 1.3, 0.9, tri, nrm, src, ssl, None, tg, (0, 0, 1, 1))
+    fmm_case("fmm_dl_only", 2, 1.3, 0.9, tri, nrm, src, None, sdl, np.zeros((0, 3)), (1, 1, 0, 0))
+    tri, nrm, src, ssl, sdl, tg = sphere_case(12, 1000, 1)
+    fmm_case("fmm_c1", 2, 1.0, 1.0, tri, nrm, src, ssl, sdl, tg)
+    fmm_case("fmm_c1_p1", 1, 1.0, 1.0, tri, nrm, src, ssl, sdl, tg)
+    t2, n2, s2, sl2, dl2, tg2 = sphere_case(2, 10, 5)
+    fmm_case("fmm_tiny", 2, 2.0, 0.7, t2[:40], n2[:40], s2[:40], sl2[:40], dl2[:40], tg2)
+
+    # pair quadratures
+    rng = np.random.default_rng(7)
+    P = {k: [] for k in ("tri", "sigma", "target", "ifself", "lam", "mu", "pt_sl", "st_sl",
+                         "pt_dl", "st_dl")}
+    for t in range(500):
+        tr = rng.uniform(-1, 1, (3, 3))
+        nv = np.cross(tr[1] - tr[0], tr[2] - tr[0])
+        nv /= np.linalg.norm(nv)
+        sg = rng.uniform(-1, 1, 3)
+        kind = t % 4
+        ifself = 0
+        if kind == 0:
+            x = tr.mean(0)
+            ifself = 1
+        elif kind == 1:
+            x = rng.uniform(-2, 2, 3)
+        elif kind == 2:
+            x = tr.mean(0) + rng.uniform(-1, 1, 3) * 0.7
+            x = x - nv * np.dot(x - tr[0], nv) + nv * rng.choice([1e-3, 1e-6, 0.0])
+        else:
+            x = tr[0] + rng.uniform(-0.05, 0.05, 3)
+        lam, mu = rng.uniform(0.5, 3), rng.uniform(0.5, 3)
+        a, b = ref.pair_sl(lam, mu, tr, sg, x, ifself)
+        c, d = ref.pair_dl(lam, mu, tr, sg, nv, x, ifself)
+        for k, v in zip(P, (tr, sg, x, ifself, lam, mu, a, b, c, d)):
+            P[k].append(v)
+    save("pairs", **{k: np.array(v) for k, v in P.items()})
+
+    # plane-triangle tables
+    T = {k: [] for k in ("geom", "iself", "btab", "rtab")}
+    for t in range(200):
+        a, b, c = rng.uniform(0.5, 2), rng.uniform(-1, 2), rng.uniform(0.3, 2)
+        if t % 3 == 0:
+            x0, y0, z0, iself = (a + b) / 3, c / 3, 0.0, 1
+        else:
+            x0, y0 = rng.uniform(-3, 4, 2)
+            z0 = rng.uniform(-2, 2) * rng.choice([1, 1e-3])
+            iself = 0
+        iquad = 0 if iself else (1 if z0 > 0 else (-1 if z0 < 0 else 0))
+        v1, v2, v3 = np.array([0.0, 0.0]), np.array([a, 0.0]), np.array([b, c])
+        bt = np.zeros((5, 5, 5))
+        rt = np.zeros((3, 3, 3))
+        for i in range(5):
+            for j in range(5 - i):
+                for k in range(5 - i - j):
+                    if i + j + k >= 2:
+                        bt[i, j, k] = ref.triabtable(i, j, k, iquad, v1, v2, v3, x0, y0, z0)
+        for i in range(3):
+            for j in range(3 - i):
+                for k in range(3 - i - j):
+                    rt[i, j, k] = ref.triartable(i, j, k, iquad, v1, v2, v3, x0, y0, z0)
+        T["geom"].append((a, b, c, x0, y0, z0))
+        T["iself"].append(iself)
+        T["btab"].append(bt)
+        T["rtab"].append(rt)
+    save("tables", **{k: np.array(v) for k, v in T.items()})
+
+    # tree and lists
+    tri, nrm, src, ssl, sdl, tg = sphere_case(12, 1000, 1)
+    tr = ref.tree(src, tg, 40)
+    nb = tr["boxes"].shape[0]
+    lists = {}
+    for t in range(1, 6):
+        start = np.zeros(nb + 1, dtype=np.int32)
+        start[1:] = np.cumsum([len(x) for x in tr["lists"][t]])
+        lists[f"lstart{t}"] = start
+        lists[f"litems{t}"] = np.concatenate(tr["lists"][t]).astype(np.int32) if start[-1] else np.zeros(0, np.int32)
+    save("tree_c1", source=src, target=tg, nbox=40, boxes=tr["boxes"], centers=tr["centers"],
+         iz=tr["iz"], iztarg=tr["iztarg"], laddr=tr["laddr"], nlev=tr["nlev"], center=tr["center"],
+         size=tr["size"], **lists)
+
+    # direct sum
+    tri, nrm, src, ssl, sdl, tg = sphere_case(4, 50, 3, 0.5, 3.0)
+    pt, st, ptt, stt = ref.el3dtriadirecttarg(1.1, 0.8, tri, nrm, src, ssl, sdl, tg)
+    save("direct_s4", lam=1.1, mu=0.8, triangle=tri, trinorm=nrm, source=src, sigma_sl=ssl,
+         sigma_dl=sdl, target=tg, ptfrc=pt, strain=st, ptfrctarg=ptt, straintarg=stt)
+
+
+if __name__ == "__main__":
+    main()

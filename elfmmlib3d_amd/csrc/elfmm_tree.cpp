@@ -92,63 +92,47 @@ void list2_orders(double eps, int *itable)
 
 namespace {
 
-// Stable split of idx by coordinate dim: coord <= thresh first
-// (src/d3tstrcr.f:1867-1897).
-int split_axis(const double *pts, int *idx, int n, int dim, double thresh, std::vector<int> &hi)
-{
-    hi.clear();
-    int lo = 0;
-    for (int i = 0; i < n; i++) {
-        int j = idx[i];
-        if (pts[3 * j + dim] > thresh)
-            hi.push_back(j);
-        else
-            idx[lo++] = j;
-    }
-    std::copy(hi.begin(), hi.end(), idx + lo);
-    return lo;
-}
-
-// Eight octants in the order z, y, x (src/d3tstrcr.f:1712-1860); octant
-// o = 4*zhi + 2*yhi + xhi. Returns counts per octant (starts are prefix sums).
-void split_octants(const double *pts, int *idx, int n, const double *c, int cnt[8],
-                   std::vector<int> &tmp)
-{
-    int nz = split_axis(pts, idx, n, 2, c[2], tmp);
-    int half[2] = {nz, n - nz};
-    int off[2] = {0, nz};
-    int q[4], qoff[4];
-    for (int h = 0; h < 2; h++) {
-        int ny = half[h] ? split_axis(pts, idx + off[h], half[h], 1, c[1], tmp) : 0;
-        q[2 * h] = ny;
-        q[2 * h + 1] = half[h] - ny;
-        qoff[2 * h] = off[h];
-        qoff[2 * h + 1] = off[h] + ny;
-    }
-    for (int k = 0; k < 4; k++) {
-        int nx = q[k] ? split_axis(pts, idx + qoff[k], q[k], 0, c[0], tmp) : 0;
-        cnt[2 * k] = nx;
-        cnt[2 * k + 1] = q[k] - nx;
-    }
-}
-
-struct Box {
-    double lo[3], hi[3];
+struct Pt {
+    double x[3];
+    int id;
 };
 
-// touch test with tolerance min(side)/1e4 (src/d3tstrcr.f:1082-1179)
-bool touching(const Box &a, const Box &b)
+// Eight octants of one box (src/d3tstrcr.f:1712-1897): the reference splits
+// by z, then y, then x with stable two-way partitions, a coordinate equal
+// to the centre going low. That order is a stable sort by the octant key
+// o = 4*(z>c) + 2*(y>c) + (x>c), done here as one counting pass over the
+// box's contiguous point records. Returns the per-octant counts.
+void split_octants(Pt *pts, int n, const double *c, int cnt[8], std::vector<Pt> &tmp)
 {
-    double eps = a.hi[0] - a.lo[0];
-    eps = std::min(eps, b.hi[0] - b.lo[0]);
-    eps = std::min(eps, b.hi[1] - b.lo[1]);
-    eps = std::min(eps, b.hi[2] - b.lo[2]);
-    eps = std::min(eps, a.hi[1] - a.lo[1]);
-    eps = std::min(eps, a.hi[2] - a.lo[2]);
-    eps = eps / 10000;
+    for (int o = 0; o < 8; o++) cnt[o] = 0;
+    if (n == 0) return;
+    tmp.resize(n);
+    std::vector<unsigned char> key(n);
+    for (int i = 0; i < n; i++) {
+        const double *x = pts[i].x;
+        int o = (x[2] > c[2] ? 4 : 0) | (x[1] > c[1] ? 2 : 0) | (x[0] > c[0] ? 1 : 0);
+        key[i] = (unsigned char)o;
+        cnt[o]++;
+    }
+    int pos[8];
+    pos[0] = 0;
+    for (int o = 1; o < 8; o++) pos[o] = pos[o - 1] + cnt[o - 1];
+    for (int i = 0; i < n; i++) tmp[pos[key[i]]++] = pts[i];
+    std::copy(tmp.begin(), tmp.end(), pts);
+}
+
+// Two boxes touch or overlap (src/d3tstrcr.f:1082-1179). The reference
+// compares floating corners with a tolerance of min(side)/1e4; corners of
+// the same cube grid differ only by rounding, so the test is exactly the
+// closed-interval test on integer box coordinates at the finer level.
+inline bool touching(const int *ijk_a, int la, const int *ijk_b, int lb)
+{
+    int L = la > lb ? la : lb;
+    int sa = L - la, sb = L - lb;
     for (int d = 0; d < 3; d++) {
-        if (a.lo[d] > b.hi[d] + eps) return false;
-        if (b.lo[d] > a.hi[d] + eps) return false;
+        long lo_a = (long)(ijk_a[d] - 1) << sa, hi_a = (long)ijk_a[d] << sa;
+        long lo_b = (long)(ijk_b[d] - 1) << sb, hi_b = (long)ijk_b[d] << sb;
+        if (lo_a > hi_b || lo_b > hi_a) return false;
     }
     return true;
 }
@@ -181,10 +165,10 @@ int build_tree(const double *src, int n, const double *targ, int nt, int nbox, i
     }
     T.size = size;
 
-    T.iz.resize(n);
-    T.iztarg.resize(nt);
-    for (int i = 0; i < n; i++) T.iz[i] = i;
-    for (int i = 0; i < nt; i++) T.iztarg[i] = i;
+    // point records in tree order; permutations are read back at the end
+    std::vector<Pt> ps(n), pt(nt);
+    for (int i = 0; i < n; i++) ps[i] = Pt{{src[3 * i], src[3 * i + 1], src[3 * i + 2]}, i};
+    for (int i = 0; i < nt; i++) pt[i] = Pt{{targ[3 * i], targ[3 * i + 1], targ[3 * i + 2]}, i};
     T.level = {0};
     T.ijk = {1, 1, 1};
     T.parent = {-1};
@@ -197,8 +181,7 @@ int build_tree(const double *src, int n, const double *targ, int nt, int nbox, i
     T.laddr = {0, 1};
     T.nlev = 0;
 
-    std::vector<int> tmp;
-    tmp.reserve(n + nt);
+    std::vector<Pt> tmp;
     for (int lev = 0; lev < 100; lev++) {
         int first = T.laddr[2 * lev], count = T.laddr[2 * lev + 1];
         int nnew = 0;
@@ -210,13 +193,13 @@ int build_tree(const double *src, int n, const double *targ, int nt, int nbox, i
             for (int d = 0; d < 3; d++)
                 c[d] = T.center[d] - size / 2 + (T.ijk[3 * b + d] - 1) * side + side2;
             int cs[8], ct[8];
-            split_octants(src, T.iz.data() + T.src_start[b], T.src_count[b], c, cs, tmp);
-            split_octants(targ, T.iztarg.data() + T.tgt_start[b], T.tgt_count[b], c, ct, tmp);
-            int ps = T.src_start[b], pt = T.tgt_start[b];
+            split_octants(ps.data() + T.src_start[b], T.src_count[b], c, cs, tmp);
+            split_octants(pt.data() + T.tgt_start[b], T.tgt_count[b], c, ct, tmp);
+            int s0 = T.src_start[b], t0 = T.tgt_start[b];
             for (int o = 0; o < 8; o++) {
-                int s0 = ps, t0 = pt;
-                ps += cs[o];
-                pt += ct[o];
+                int sa = s0, ta = t0;
+                s0 += cs[o];
+                t0 += ct[o];
                 if (cs[o] == 0 && ct[o] == 0) continue;   // empty octants make no box
                 int kid = (int)T.level.size();
                 T.level.push_back(lev + 1);
@@ -226,9 +209,9 @@ int build_tree(const double *src, int n, const double *targ, int nt, int nbox, i
                 T.parent.push_back(b);
                 for (int q = 0; q < 8; q++) T.kids.push_back(-1);
                 T.nkids.push_back(0);
-                T.src_start.push_back(s0);
+                T.src_start.push_back(sa);
                 T.src_count.push_back(cs[o]);
-                T.tgt_start.push_back(t0);
+                T.tgt_start.push_back(ta);
                 T.tgt_count.push_back(ct[o]);
                 T.kids[8 * b + T.nkids[b]] = kid;   // compacted child slots
                 T.nkids[b]++;
@@ -240,26 +223,30 @@ int build_tree(const double *src, int n, const double *targ, int nt, int nbox, i
         T.laddr.push_back(nnew);
         T.nlev = lev + 1;
     }
+    T.iz.resize(n);
+    T.iztarg.resize(nt);
+    for (int i = 0; i < n; i++) T.iz[i] = ps[i].id;
+    for (int i = 0; i < nt; i++) T.iztarg[i] = pt[i].id;
     int nb = (int)T.level.size();
     T.nboxes = nb;
 
-    // exact centres and cube extents of d3tcentc (src/d3tstrcr.f:1596-1676)
+    // exact centres of d3tcentc (src/d3tstrcr.f:1596-1676)
     T.centers.resize(3 * nb);
-    std::vector<Box> ext(nb);
     for (int b = 0; b < nb; b++) {
         double side = size / (double)(1 << T.level[b]), side2 = side / 2;
-        for (int d = 0; d < 3; d++) {
-            double c = T.center[d] - size / 2 + (T.ijk[3 * b + d] - 1) * side + side2;
-            T.centers[3 * b + d] = c;
-            ext[b].lo[d] = c - side / 2;
-            ext[b].hi[d] = ext[b].lo[d] + side;
-        }
+        for (int d = 0; d < 3; d++)
+            T.centers[3 * b + d] = T.center[d] - size / 2 + (T.ijk[3 * b + d] - 1) * side + side2;
     }
+    auto touch = [&](int a, int b) { return touching(&T.ijk[3 * a], T.level[a], &T.ijk[3 * b], T.level[b]); };
 
     // lists 5 (colleagues) and 2 (well separated children of the parent's
     // colleagues), src/d3tstrcr.f:760-828
     std::vector<std::vector<int>> L[6];
     for (int t = 1; t <= 5; t++) L[t].assign(nb, {});
+    for (int b = 0; b < nb; b++) {
+        L[5][b].reserve(26);
+        L[2][b].reserve(64);
+    }
     for (int b = 1; b < nb; b++) {
         int dad = T.parent[b];
         int ncoll = 1 + (int)L[5][dad].size();
@@ -268,7 +255,7 @@ int build_tree(const double *src, int n, const double *targ, int nt, int nbox, i
             for (int q = 0; q < T.nkids[coll]; q++) {
                 int kid = T.kids[8 * coll + q];
                 if (kid == b) continue;
-                if (touching(ext[kid], ext[b]))
+                if (touch(kid, b))
                     L[5][b].push_back(kid);
                 else
                     L[2][b].push_back(kid);
@@ -282,6 +269,7 @@ int build_tree(const double *src, int n, const double *targ, int nt, int nbox, i
         int box, left;
     };
     std::vector<Frame> stack;
+    stack.reserve(128);
     for (int b = 0; b < nb; b++) {
         if (T.nkids[b] > 0 || T.level[b] == 0) continue;
         for (int c0 : L[5][b]) {
@@ -290,7 +278,7 @@ int build_tree(const double *src, int n, const double *targ, int nt, int nbox, i
             while (!stack.empty()) {
                 Frame &f = stack.back();
                 int j = f.box;
-                if (!touching(ext[b], ext[j])) {
+                if (!touch(b, j)) {
                     L[3][b].push_back(j);
                     stack.pop_back();
                     stack.back().left--;

@@ -75,7 +75,10 @@ struct EdgePrims {
     double k01, k11, k03, k13, k23, k05, k15, k25, k35;
 };
 
-// int_{t1}^{t2} t^m / (t^2 + rho2)^(q/2) dt for the (m,q) pairs above.
+// int_{t1}^{t2} t^m / (t^2 + rho2)^(q/2) dt. LOW selects the primitives of
+// the orders <= 3 tables (q in {1,3}); otherwise those of the order-4 tables
+// (q in {3,5}), which need no logarithm.
+template <bool LOW>
 ELFMM_HD void edge_prims(double t1, double t2, double rho2, EdgePrims &p)
 {
     bool flip = t2 <= 0.0;
@@ -88,29 +91,33 @@ ELFMM_HD void edge_prims(double t1, double t2, double rho2, EdgePrims &p)
     double k11 = (t2 - t1) * (t2 + t1) / (r1 + r2);
     double ir1 = 1.0 / r1, ir2 = 1.0 / r2;
     double k13 = k11 * ir1 * ir2;
-    double ir13 = ir1 * ir1 * ir1, ir23 = ir2 * ir2 * ir2;
-    double k15 = k11 * (r2 * r2 + r1 * r2 + r1 * r1) * ir13 * ir23 * (1.0 / 3.0);
-    double k01, k03, k05, k25;
+    double k01 = 0.0, k03, k05 = 0.0, k15 = 0.0, k25 = 0.0;
+    double r13 = r1 * r1 * r1, r23 = r2 * r2 * r2;
+    if (!LOW) k15 = k11 * (r2 * r2 + r1 * r2 + r1 * r1) * (ir1 * ir1 * ir1) * (ir2 * ir2 * ir2) * (1.0 / 3.0);
     if (t1 >= 0.0) {
-        k01 = log((t2 + r2) / (t1 + r1));
+        if (LOW) k01 = log((t2 + r2) / (t1 + r1));
         k03 = 1.0 / ((t1 + r1) * r1) - 1.0 / ((t2 + r2) * r2);
-        double r13 = r1 * r1 * r1, r23 = r2 * r2 * r2;
-        double ph1 = (3.0 * t1 * t1 + 4.0 * rho2) / ((2.0 * t1 * t1 * t1 + 3.0 * rho2 * t1 + 2.0 * r13) * r13);
-        double ph2 = (3.0 * t2 * t2 + 4.0 * rho2) / ((2.0 * t2 * t2 * t2 + 3.0 * rho2 * t2 + 2.0 * r23) * r23);
-        k05 = (ph1 - ph2) * (1.0 / 3.0);
-        double c1 = (3.0 * t1 * t1 * t1 * t1 + 3.0 * t1 * t1 * rho2 + rho2 * rho2) / ((t1 * t1 * t1 + r13) * r13);
-        double c2 = (3.0 * t2 * t2 * t2 * t2 + 3.0 * t2 * t2 * rho2 + rho2 * rho2) / ((t2 * t2 * t2 + r23) * r23);
-        k25 = (c1 - c2) * (1.0 / 3.0);
+        if (!LOW) {
+            double ph1 = (3.0 * t1 * t1 + 4.0 * rho2) / ((2.0 * t1 * t1 * t1 + 3.0 * rho2 * t1 + 2.0 * r13) * r13);
+            double ph2 = (3.0 * t2 * t2 + 4.0 * rho2) / ((2.0 * t2 * t2 * t2 + 3.0 * rho2 * t2 + 2.0 * r23) * r23);
+            k05 = (ph1 - ph2) * (1.0 / 3.0);
+            double c1 = (3.0 * t1 * t1 * t1 * t1 + 3.0 * t1 * t1 * rho2 + rho2 * rho2) / ((t1 * t1 * t1 + r13) * r13);
+            double c2 = (3.0 * t2 * t2 * t2 * t2 + 3.0 * t2 * t2 * rho2 + rho2 * rho2) / ((t2 * t2 * t2 + r23) * r23);
+            k25 = (c1 - c2) * (1.0 / 3.0);
+        }
     } else {
         double irho2 = 1.0 / rho2;
-        k01 = log((t2 + r2) * (r1 - t1) * irho2);
+        if (LOW) k01 = log((t2 + r2) * (r1 - t1) * irho2);
         k03 = (t2 * ir2 - t1 * ir1) * irho2;
-        k05 = (t2 * (2.0 * t2 * t2 + 3.0 * rho2) * ir23 - t1 * (2.0 * t1 * t1 + 3.0 * rho2) * ir13) *
-              (irho2 * irho2 * (1.0 / 3.0));
-        k25 = (t2 * t2 * t2 * ir23 - t1 * t1 * t1 * ir13) * (irho2 * (1.0 / 3.0));
+        if (!LOW) {
+            double ir13 = ir1 * ir1 * ir1, ir23 = ir2 * ir2 * ir2;
+            k05 = (t2 * (2.0 * t2 * t2 + 3.0 * rho2) * ir23 - t1 * (2.0 * t1 * t1 + 3.0 * rho2) * ir13) *
+                  (irho2 * irho2 * (1.0 / 3.0));
+            k25 = (t2 * t2 * t2 * ir23 - t1 * t1 * t1 * ir13) * (irho2 * (1.0 / 3.0));
+        }
     }
-    double k23 = k01 - rho2 * k03;
-    double k35 = k13 - rho2 * k15;
+    double k23 = LOW ? k01 - rho2 * k03 : 0.0;
+    double k35 = LOW ? 0.0 : k13 - rho2 * k15;
     if (flip) {
         k11 = -k11;
         k13 = -k13;
@@ -122,99 +129,77 @@ ELFMM_HD void edge_prims(double t1, double t2, double rho2, EdgePrims &p)
 }
 
 // Symmetric tensors: order-2 components xx xy xz yy yz zz, order-3
-// components xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz (lexicographic).
-struct Tables {
+// components xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz, order-4 xxxx xxxy xxxz
+// xxyy xxyz xxzz xyyy xyyz xyzz xzzz yyyy yyyz yyzz yzzz zzzz.
+struct TablesLow {
     double F, r1[3];      // int 1/r; int d_i (1/r)
-    double r2[6];         // int d_i d_j (1/r)
     double b2[6];         // int d_i d_j r
     double b3[10];        // int d_i d_j d_k r
-    double b4[15];        // int d_i d_j d_k d_l r, order xxxx xxxy xxxz xxyy xxyz xxzz xyyy xyyz
-                          //   xyzz xzzz yyyy yyyz yyzz yzzz zzzz
+};
+struct TablesHigh {
+    double r2[6];         // int d_i d_j (1/r)
+    double b4[15];        // int d_i d_j d_k d_l r
 };
 
-// Tables for a target at (x0,y0,z0) of a triangle (0,0),(a,0),(b,c) in the
-// reference's derivative convention (derivatives with respect to the
-// source point, src/triahquad.f:680-1014). iself selects the principal
-// value for the triangle's own centroid.
-ELFMM_HD void nf_tables(double a, double b, double c, double x0, double y0, double z0,
-                        int iself, Tables &T)
+// Geometry of edge e of the triangle (0,0),(a,0),(b,c) seen from (x0,y0):
+// unit tangent s, outward normal n, foot parameter t1 and signed distance.
+ELFMM_HD void nf_edge(int e, double a, double b, double c, double x0, double y0, double &sx,
+                      double &sy, double &L, double &t1, double &del)
+{
+    double ax = e == 0 ? 0.0 : (e == 1 ? a : b), ay = e == 2 ? c : 0.0;
+    double bx = e == 0 ? a : (e == 1 ? b : 0.0), by = e == 1 ? c : 0.0;
+    double dx = bx - ax, dy = by - ay;
+    L = sqrt(dx * dx + dy * dy);
+    double iL = 1.0 / L;
+    sx = dx * iL;
+    sy = dy * iL;
+    // outward normal (sy, -sx)
+    t1 = (ax - x0) * sx + (ay - y0) * sy;
+    del = (ax - x0) * sy - (ay - y0) * sx;
+}
+
+// Orders <= 3 of r and <= 1 of 1/r, in the reference's derivative
+// convention (derivatives with respect to the source point,
+// src/triahquad.f:680-1014). iself selects the principal value for the
+// triangle's own centroid.
+ELFMM_HD void nf_tables_low(double a, double b, double c, double x0, double y0, double z0, int iself,
+                            TablesLow &T)
 {
     double w = iself ? 0.0 : -z0;
     double zz = iself ? 0.0 : z0;
-    double vx[3] = {0.0, a, b}, vy[3] = {0.0, 0.0, c};
     double sumF = 0.0, r1x = 0.0, r1y = 0.0;
-    double e1[2][3] = {{0, 0, 0}, {0, 0, 0}};
-    double b1[2][3] = {{0, 0, 0}, {0, 0, 0}};
-    double d2s[2][6], d3s[2][10];
-    for (int q = 0; q < 2; q++) {
-        for (int i = 0; i < 6; i++) d2s[q][i] = 0.0;
-        for (int i = 0; i < 10; i++) d3s[q][i] = 0.0;
-    }
-#pragma unroll
+    double b1x[3] = {0, 0, 0}, b1yy = 0.0, b1yz = 0.0;      // n_x E[D1 r], n_y E[D1 r]_{y,z}
+    double d2x[6] = {0, 0, 0, 0, 0, 0}, d2y[3] = {0, 0, 0}; // n_x E[D2 r], n_y E[D2 r]_{yy,yz,zz}
+#pragma nounroll
     for (int e = 0; e < 3; e++) {
-        double ax = vx[e], ay = vy[e];
-        double bx = vx[e == 2 ? 0 : e + 1], by = vy[e == 2 ? 0 : e + 1];
-        double dx = bx - ax, dy = by - ay;
-        double L = sqrt(dx * dx + dy * dy);
-        double iL = 1.0 / L;
-        double sx = dx * iL, sy = dy * iL;
-        double nx = sy, ny = -sx;                 // outward normal
-        double t1 = (ax - x0) * sx + (ay - y0) * sy;
-        double del = (ax - x0) * nx + (ay - y0) * ny;
+        double sx, sy, L, t1, del;
+        nf_edge(e, a, b, c, x0, y0, sx, sy, L, t1, del);
+        double nx = sy, ny = -sx;
         EdgePrims p;
-        edge_prims(t1, t1 + L, del * del + w * w, p);
+        edge_prims<true>(t1, t1 + L, del * del + w * w, p);
         double s[3] = {sx, sy, 0.0};
         double v[3] = {del * nx, del * ny, w};
         sumF += del * p.k01;
         r1x += nx * p.k01;
         r1y += ny * p.k01;
-        double U3[3], U1[3];
+        double U1[3];
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            U3[i] = s[i] * p.k13 + v[i] * p.k03;
-            U1[i] = s[i] * p.k11 + v[i] * p.k01;
-        }
-        double D2[6], D3[10];
-        {
-            int idx = 0;
+        for (int i = 0; i < 3; i++) U1[i] = s[i] * p.k11 + v[i] * p.k01;
 #pragma unroll
-            for (int i = 0; i < 3; i++)
+        for (int i = 0; i < 3; i++) b1x[i] += nx * U1[i];
+        b1yy += ny * U1[1];
+        b1yz += ny * U1[2];
+        int idx = 0;
 #pragma unroll
-                for (int j = i; j < 3; j++) {
-                    double uu = s[i] * s[j] * p.k23 + (s[i] * v[j] + v[i] * s[j]) * p.k13 + v[i] * v[j] * p.k03;
-                    D2[idx++] = (i == j ? p.k01 : 0.0) - uu;
-                }
-        }
-        {
-            int idx = 0;
+        for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = i; j < 3; j++)
-#pragma unroll
-                    for (int k = j; k < 3; k++) {
-                        double sss = s[i] * s[j] * s[k];
-                        double ssv = s[i] * s[j] * v[k] + s[i] * v[j] * s[k] + v[i] * s[j] * s[k];
-                        double svv = s[i] * v[j] * v[k] + v[i] * s[j] * v[k] + v[i] * v[j] * s[k];
-                        double vvv = v[i] * v[j] * v[k];
-                        double uuu = sss * p.k35 + ssv * p.k25 + svv * p.k15 + vvv * p.k05;
-                        double tr = (i == j ? U3[k] : 0.0) + (i == k ? U3[j] : 0.0) + (j == k ? U3[i] : 0.0);
-                        D3[idx++] = 3.0 * uuu - tr;
-                    }
-        }
-        double n2[2] = {nx, ny};
-#pragma unroll
-        for (int q = 0; q < 2; q++) {
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                e1[q][i] -= n2[q] * U3[i];
-                b1[q][i] += n2[q] * U1[i];
+            for (int j = i; j < 3; j++) {
+                double uu = s[i] * s[j] * p.k23 + (s[i] * v[j] + v[i] * s[j]) * p.k13 + v[i] * v[j] * p.k03;
+                double d2 = (i == j ? p.k01 : 0.0) - uu;
+                d2x[idx] += nx * d2;
+                if (idx >= 3) d2y[idx - 3] += ny * d2;
+                idx++;
             }
-#pragma unroll
-            for (int i = 0; i < 6; i++) d2s[q][i] += n2[q] * D2[i];
-#pragma unroll
-            for (int i = 0; i < 10; i++) d3s[q][i] += n2[q] * D3[i];
-        }
     }
     double wz = 0.0;
     if (!(iself || z0 == 0.0)) {
@@ -234,47 +219,78 @@ ELFMM_HD void nf_tables(double a, double b, double c, double x0, double y0, doub
     T.r1[0] = r1x;
     T.r1[1] = r1y;
     T.r1[2] = wz;
-    // order 2 of 1/r: xx xy xz yy yz zz
-    T.r2[0] = e1[0][0];
-    T.r2[1] = e1[0][1];
-    T.r2[2] = e1[0][2];
-    T.r2[3] = e1[1][1];
-    T.r2[4] = e1[1][2];
-    T.r2[5] = -T.r2[0] - T.r2[3];
-    // order 2 of r
-    T.b2[0] = b1[0][0];
-    T.b2[1] = b1[0][1];
-    T.b2[2] = b1[0][2];
-    T.b2[3] = b1[1][1];
-    T.b2[4] = b1[1][2];
+    T.b2[0] = b1x[0];
+    T.b2[1] = b1x[1];
+    T.b2[2] = b1x[2];
+    T.b2[3] = b1yy;
+    T.b2[4] = b1yz;
     T.b2[5] = 2.0 * T.F - T.b2[0] - T.b2[3];
-    // order 3 of r: x-leading from n_x, the y/z-only ones from n_y
-    T.b3[0] = d2s[0][0];   // xxx <- n_x * xx
-    T.b3[1] = d2s[0][1];   // xxy <- n_x * xy
-    T.b3[2] = d2s[0][2];   // xxz
-    T.b3[3] = d2s[0][3];   // xyy
-    T.b3[4] = d2s[0][4];   // xyz
-    T.b3[5] = d2s[0][5];   // xzz
-    T.b3[6] = d2s[1][3];   // yyy <- n_y * yy
-    T.b3[7] = d2s[1][4];   // yyz
-    T.b3[8] = d2s[1][5];   // yzz
-    T.b3[9] = 2.0 * T.r1[2] - T.b3[2] - T.b3[7];   // zzz
-    // order 4 of r
-    T.b4[0] = d3s[0][0];   // xxxx
-    T.b4[1] = d3s[0][1];   // xxxy
-    T.b4[2] = d3s[0][2];   // xxxz
-    T.b4[3] = d3s[0][3];   // xxyy
-    T.b4[4] = d3s[0][4];   // xxyz
-    T.b4[5] = d3s[0][5];   // xxzz
-    T.b4[6] = d3s[0][6];   // xyyy
-    T.b4[7] = d3s[0][7];   // xyyz
-    T.b4[8] = d3s[0][8];   // xyzz
-    T.b4[9] = d3s[0][9];   // xzzz
-    T.b4[10] = d3s[1][6];  // yyyy
-    T.b4[11] = d3s[1][7];  // yyyz
-    T.b4[12] = d3s[1][8];  // yyzz
-    T.b4[13] = d3s[1][9];  // yzzz
-    T.b4[14] = 2.0 * T.r2[5] - T.b4[5] - T.b4[12];   // zzzz
+    // x-leading entries from n_x, the y/z-only ones from n_y
+#pragma unroll
+    for (int i = 0; i < 6; i++) T.b3[i] = d2x[i];
+    T.b3[6] = d2y[0];
+    T.b3[7] = d2y[1];
+    T.b3[8] = d2y[2];
+    T.b3[9] = 2.0 * T.r1[2] - T.b3[2] - T.b3[7];
+}
+
+// Order 2 of 1/r and order 4 of r.
+ELFMM_HD void nf_tables_high(double a, double b, double c, double x0, double y0, double z0, int iself,
+                             TablesHigh &T)
+{
+    double w = iself ? 0.0 : -z0;
+    double e1x[3] = {0, 0, 0}, e1yy = 0.0, e1yz = 0.0;
+    double d3x[10], d3y[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 10; i++) d3x[i] = 0.0;
+#pragma nounroll
+    for (int e = 0; e < 3; e++) {
+        double sx, sy, L, t1, del;
+        nf_edge(e, a, b, c, x0, y0, sx, sy, L, t1, del);
+        double nx = sy, ny = -sx;
+        EdgePrims p;
+        edge_prims<false>(t1, t1 + L, del * del + w * w, p);
+        double s[3] = {sx, sy, 0.0};
+        double v[3] = {del * nx, del * ny, w};
+        double U3[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) U3[i] = s[i] * p.k13 + v[i] * p.k03;
+#pragma unroll
+        for (int i = 0; i < 3; i++) e1x[i] -= nx * U3[i];
+        e1yy -= ny * U3[1];
+        e1yz -= ny * U3[2];
+        int idx = 0;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = i; j < 3; j++)
+#pragma unroll
+                for (int k = j; k < 3; k++) {
+                    double sss = s[i] * s[j] * s[k];
+                    double ssv = s[i] * s[j] * v[k] + s[i] * v[j] * s[k] + v[i] * s[j] * s[k];
+                    double svv = s[i] * v[j] * v[k] + v[i] * s[j] * v[k] + v[i] * v[j] * s[k];
+                    double vvv = v[i] * v[j] * v[k];
+                    double uuu = sss * p.k35 + ssv * p.k25 + svv * p.k15 + vvv * p.k05;
+                    double tr = (i == j ? U3[k] : 0.0) + (i == k ? U3[j] : 0.0) + (j == k ? U3[i] : 0.0);
+                    double d3 = 3.0 * uuu - tr;
+                    d3x[idx] += nx * d3;
+                    if (idx >= 6) d3y[idx - 6] += ny * d3;
+                    idx++;
+                }
+    }
+    T.r2[0] = e1x[0];
+    T.r2[1] = e1x[1];
+    T.r2[2] = e1x[2];
+    T.r2[3] = e1yy;
+    T.r2[4] = e1yz;
+    T.r2[5] = -T.r2[0] - T.r2[3];
+#pragma unroll
+    for (int i = 0; i < 10; i++) T.b4[i] = d3x[i];
+    T.b4[10] = d3y[0];
+    T.b4[11] = d3y[1];
+    T.b4[12] = d3y[2];
+    T.b4[13] = d3y[3];
+    T.b4[14] = 2.0 * T.r2[5] - T.b4[5] - T.b4[12];
 }
 
 // component index helpers for derivative multi-indices
@@ -318,8 +334,10 @@ ELFMM_HD int nf_i4(int i, int j, int k, int l)
 }
 
 // One (target, triangle) pair: accumulate displacement (3) and strain
-// (xx yy zz xy xz yz) in global coordinates into out[9]. coefficients
-// c1 = (lam+mu)/(lam+2mu); ifsl/ifdl select the layers.
+// (xx yy zz xy xz yz) in global coordinates into out[9]; c1 =
+// (lam+mu)/(lam+2mu); ifsl/ifdl select the layers. Two passes over the
+// edges keep the live state small: orders <= 3 (single layer, double-layer
+// displacement), then order 4 (double-layer strain).
 ELFMM_HD void nf_pair(const double *rec, const double *P, int iself, double rlam, double rmu,
                       double c1, int ifsl, int ifdl, double *out)
 {
@@ -328,88 +346,76 @@ ELFMM_HD void nf_pair(const double *rec, const double *P, int iself, double rlam
     double x0 = U[0] * q0 + U[1] * q1 + U[2] * q2;
     double y0 = U[3] * q0 + U[4] * q1 + U[5] * q2;
     double z0 = U[6] * q0 + U[7] * q1 + U[8] * q2;
-    Tables T;
-    nf_tables(rec[12], rec[13], rec[14], x0, y0, z0, iself, T);
     double disp[3] = {0, 0, 0};
     double H[6] = {0, 0, 0, 0, 0, 0};   // local xx xy xz yy yz zz
-    if (ifsl) {
-        const double *sg = rec + 15;
-        // disp_i = -c1 sum_k s_k b(i,k) + 2 s_i F
+    {
+        TablesLow T;
+        nf_tables_low(rec[12], rec[13], rec[14], x0, y0, z0, iself, T);
+        if (ifsl) {
+            const double *sg = rec + 15;
+            // disp_i = -c1 sum_k s_k b(i,k) + 2 s_i F
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            double acc = 0.0;
-#pragma unroll
-            for (int k = 0; k < 3; k++) acc += sg[k] * T.b2[nf_i2(i, k)];
-            disp[i] += -c1 * acc + 2.0 * sg[i] * T.F;
-        }
-        // strain_ij = -[ -c1 sum_k s_k b(i,j,k) + s_i r(j) + s_j r(i) ]
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = i; j < 3; j++) {
+            for (int i = 0; i < 3; i++) {
                 double acc = 0.0;
 #pragma unroll
-                for (int k = 0; k < 3; k++) acc += sg[k] * T.b3[nf_i3(i, j, k)];
-                H[nf_i2(i, j)] -= -c1 * acc + sg[i] * T.r1[j] + sg[j] * T.r1[i];
+                for (int k = 0; k < 3; k++) acc += sg[k] * T.b2[nf_i2(i, k)];
+                disp[i] += -c1 * acc + 2.0 * sg[i] * T.F;
             }
+            // strain_ij = -[ -c1 sum_k s_k b(i,j,k) + s_i r(j) + s_j r(i) ]
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = i; j < 3; j++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 3; k++) acc += sg[k] * T.b3[nf_i3(i, j, k)];
+                    H[nf_i2(i, j)] -= -c1 * acc + sg[i] * T.r1[j] + sg[j] * T.r1[i];
+                }
+        }
+        if (ifdl) {
+            const double *sg = rec + 18;
+            // Reference assembly (src/elfmm3dtria.f:2597-2634): with
+            // u(a,k,b) = -c1 b(a,b,k) + 2 d_ak r(b) and
+            // T(a,k) = lam d_k3 sum_l u(l,a,l) + mu (u(k,a,3) + u(3,a,k)),
+            // disp_a = sum_k s_k T(a,k). Summing over k first:
+            // disp_a = lam s_z (2 r(a) - c1 sum_l b(l,l,a))
+            //        + mu (-2 c1 sum_k s_k b(a,k,z) + 2 s_a r(z) + 2 d_az (s.r))
+            double sr = sg[0] * T.r1[0] + sg[1] * T.r1[1] + sg[2] * T.r1[2];
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                double lap = T.b3[nf_i3(0, 0, a)] + T.b3[nf_i3(1, 1, a)] + T.b3[nf_i3(2, 2, a)];
+                double q = sg[0] * T.b3[nf_i3(a, 0, 2)] + sg[1] * T.b3[nf_i3(a, 1, 2)] + sg[2] * T.b3[nf_i3(a, 2, 2)];
+                disp[a] += rlam * sg[2] * (2.0 * T.r1[a] - c1 * lap) +
+                           rmu * (-2.0 * c1 * q + 2.0 * sg[a] * T.r1[2] + (a == 2 ? 2.0 * sr : 0.0));
+            }
+        }
     }
     if (ifdl) {
+        TablesHigh T;
+        nf_tables_high(rec[12], rec[13], rec[14], x0, y0, z0, iself, T);
         const double *sg = rec + 18;
-        // u(a,k,b) = -c1 b(a,b,k) + 2 d_ak r(b);  tr_k = sum_l u(l,k,l)
-        // T(a,k) = lam d_k3 tr_a + mu (u(k,a,3) + u(3,a,k))
-        double trd[3];
+        // Reference assembly (src/elfmm3dtria.f:2636-2660, 2692-2714): with
+        // d(a,k,b,m) = -c1 b(a,b,k,m) + 2 d_ak r(b,m) and
+        // S(a,k,b) = lam d_k3 sum_l d(l,a,l,b) + mu (d(k,a,3,b) + d(3,a,k,b)),
+        // strain_ab = -sym_ab sum_k s_k S(a,k,b). Summing over k first:
+        // lam s_z (2 r(a,b) - c1 sum_l b(l,l,a,b))
+        // + mu (-2 c1 sum_k s_k b(a,b,k,z) + s_a r(z,b) + s_b r(z,a)
+        //       + d_az (s.r)_b + d_bz (s.r)_a)
+        double sr[3];
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            double acc = 0.0;
-#pragma unroll
-            for (int l = 0; l < 3; l++) acc += -c1 * T.b3[nf_i3(l, l, i)] + (l == i ? 2.0 * T.r1[l] : 0.0);
-            trd[i] = acc;
-        }
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            double acc = 0.0;
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                double ukz = -c1 * T.b3[nf_i3(k, 2, a)] + (k == a ? 2.0 * T.r1[2] : 0.0);
-                double uzk = -c1 * T.b3[nf_i3(2, k, a)] + (a == 2 ? 2.0 * T.r1[k] : 0.0);
-                double Tak = (k == 2 ? rlam * trd[a] : 0.0) + rmu * (ukz + uzk);
-                acc += sg[k] * Tak;
-            }
-            disp[a] += acc;
-        }
-        // d(a,k,b,m) = -c1 b(a,b,k,m) + 2 d_ak r(b,m)
-        // S(a,k,b) = lam d_k3 sum_l d(l,a,l,b) + mu (d(k,a,3,b) + d(3,a,k,b))
+        for (int b = 0; b < 3; b++)
+            sr[b] = sg[0] * T.r2[nf_i2(0, b)] + sg[1] * T.r2[nf_i2(1, b)] + sg[2] * T.r2[nf_i2(2, b)];
 #pragma unroll
         for (int a = 0; a < 3; a++)
 #pragma unroll
             for (int b = a; b < 3; b++) {
-                double acc = 0.0;
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    double sab = 0.0, sba = 0.0;
-                    // S(a,k,b)
-                    {
-                        double tr = 0.0;
-#pragma unroll
-                        for (int l = 0; l < 3; l++)
-                            tr += -c1 * T.b4[nf_i4(l, l, a, b)] + (l == a ? 2.0 * T.r2[nf_i2(l, b)] : 0.0);
-                        double d1 = -c1 * T.b4[nf_i4(k, 2, a, b)] + (k == a ? 2.0 * T.r2[nf_i2(2, b)] : 0.0);
-                        double d2 = -c1 * T.b4[nf_i4(2, k, a, b)] + (a == 2 ? 2.0 * T.r2[nf_i2(k, b)] : 0.0);
-                        sab = (k == 2 ? rlam * tr : 0.0) + rmu * (d1 + d2);
-                    }
-                    // S(b,k,a)
-                    {
-                        double tr = 0.0;
-#pragma unroll
-                        for (int l = 0; l < 3; l++)
-                            tr += -c1 * T.b4[nf_i4(l, l, b, a)] + (l == b ? 2.0 * T.r2[nf_i2(l, a)] : 0.0);
-                        double d1 = -c1 * T.b4[nf_i4(k, 2, b, a)] + (k == b ? 2.0 * T.r2[nf_i2(2, a)] : 0.0);
-                        double d2 = -c1 * T.b4[nf_i4(2, k, b, a)] + (b == 2 ? 2.0 * T.r2[nf_i2(k, a)] : 0.0);
-                        sba = (k == 2 ? rlam * tr : 0.0) + rmu * (d1 + d2);
-                    }
-                    acc += sg[k] * 0.5 * (sab + sba);
-                }
-                H[nf_i2(a, b)] -= acc;
+                double lap = T.b4[nf_i4(0, 0, a, b)] + T.b4[nf_i4(1, 1, a, b)] + T.b4[nf_i4(2, 2, a, b)];
+                double q = sg[0] * T.b4[nf_i4(a, b, 0, 2)] + sg[1] * T.b4[nf_i4(a, b, 1, 2)] +
+                           sg[2] * T.b4[nf_i4(a, b, 2, 2)];
+                double v = rlam * sg[2] * (2.0 * T.r2[nf_i2(a, b)] - c1 * lap) +
+                           rmu * (-2.0 * c1 * q + sg[a] * T.r2[nf_i2(2, b)] + sg[b] * T.r2[nf_i2(2, a)] +
+                                  (a == 2 ? sr[b] : 0.0) + (b == 2 ? sr[a] : 0.0));
+                H[nf_i2(a, b)] -= v;
             }
     }
     // back to global coordinates: disp_g = U disp, H_g = U H U^T; scale 1/(2mu)

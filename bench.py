@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--ntarg", type=int, default=100000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", type=int, default=2000, help="targets checked against the direct sum")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="run one device copy of MIB MiB first (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -97,6 +99,12 @@ def main():
         if ier != 0:
             raise RuntimeError(f"ier={ier}")
 
+    if args.calib_copy:
+        a = torch.ones(args.calib_copy << 17, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()

@@ -141,9 +141,9 @@ def last_stats():
 
 
 def last_timings():
-    """Per-phase times (ms) of the last apply on this thread (ELFMM_TIMING=1)."""
-    out = np.zeros(12)
+    """Per-phase times (ms) of the last apply on this thread (include/elfmm_gfx950.h)."""
+    out = np.zeros(16)
     library().elfmm_gfx950_last_timings(_p(out))
     keys = ["plan", "upload", "p2m", "m2m", "m2l", "p2l", "l2l", "l2t", "m2t", "near",
-            "assemble", "total"]
+            "assemble", "total", "tree", "plan_tasks"]
     return dict(zip(keys, out.tolist()))

@@ -2,9 +2,16 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 namespace elfmm {
+
+// Host worker pool (elfmm_pool.cpp): fn(chunk, thread) is called once for
+// every chunk in [0, nchunks), chunks taken dynamically by up to
+// host_threads() threads; returns when all chunks are done.
+int host_threads();
+void parallel_chunks(int nchunks, const std::function<void(int, int)> &fn);
 
 // FMM precision flag -> tolerance and leaf size, as the reference's
 // lfmm3dtrilhesstarg (src/lfmm3dtrilhess.f:216-235): nbox = (120*2^k)/3.

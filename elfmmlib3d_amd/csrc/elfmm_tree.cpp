@@ -100,8 +100,13 @@ struct Pt {
 // Eight octants of one box (src/d3tstrcr.f:1712-1897): the reference splits
 // by z, then y, then x with stable two-way partitions, a coordinate equal
 // to the centre going low. That order is a stable sort by the octant key
-// o = 4*(z>c) + 2*(y>c) + (x>c), done here as one counting pass over the
-// box's contiguous point records. Returns the per-octant counts.
+// o = 4*(z>c) + 2*(y>c) + (x>c), done here as a counting sort over the
+// box's contiguous point records.
+inline int octant_key(const double *x, const double *c)
+{
+    return (x[2] > c[2] ? 4 : 0) | (x[1] > c[1] ? 2 : 0) | (x[0] > c[0] ? 1 : 0);
+}
+
 void split_octants(Pt *pts, int n, const double *c, int cnt[8], std::vector<Pt> &tmp)
 {
     for (int o = 0; o < 8; o++) cnt[o] = 0;
@@ -109,8 +114,7 @@ void split_octants(Pt *pts, int n, const double *c, int cnt[8], std::vector<Pt> 
     tmp.resize(n);
     std::vector<unsigned char> key(n);
     for (int i = 0; i < n; i++) {
-        const double *x = pts[i].x;
-        int o = (x[2] > c[2] ? 4 : 0) | (x[1] > c[1] ? 2 : 0) | (x[0] > c[0] ? 1 : 0);
+        int o = octant_key(pts[i].x, c);
         key[i] = (unsigned char)o;
         cnt[o]++;
     }
@@ -119,6 +123,50 @@ void split_octants(Pt *pts, int n, const double *c, int cnt[8], std::vector<Pt> 
     for (int o = 1; o < 8; o++) pos[o] = pos[o - 1] + cnt[o - 1];
     for (int i = 0; i < n; i++) tmp[pos[key[i]]++] = pts[i];
     std::copy(tmp.begin(), tmp.end(), pts);
+}
+
+// The same stable split of one large box by all pool threads: per-chunk
+// counts, offsets by (octant, chunk), scatter, copy back.
+void split_octants_par(Pt *pts, int n, const double *c, int cnt[8], std::vector<Pt> &tmp,
+                       std::vector<unsigned char> &key)
+{
+    const int CH = 8192;
+    int nch = (n + CH - 1) / CH;
+    tmp.resize(n);
+    key.resize(n);
+    std::vector<int> cc(8 * (size_t)nch, 0);
+    parallel_chunks(nch, [&](int ch, int) {
+        int a = ch * CH, e = std::min(n, a + CH);
+        int *k8 = &cc[8 * (size_t)ch];
+        for (int i = a; i < e; i++) {
+            int o = octant_key(pts[i].x, c);
+            key[i] = (unsigned char)o;
+            k8[o]++;
+        }
+    });
+    for (int o = 0; o < 8; o++) {
+        cnt[o] = 0;
+        for (int ch = 0; ch < nch; ch++) cnt[o] += cc[8 * (size_t)ch + o];
+    }
+    int base = 0;
+    for (int o = 0; o < 8; o++) {
+        int run = base;
+        for (int ch = 0; ch < nch; ch++) {
+            int v = cc[8 * (size_t)ch + o];
+            cc[8 * (size_t)ch + o] = run;
+            run += v;
+        }
+        base += cnt[o];
+    }
+    parallel_chunks(nch, [&](int ch, int) {
+        int a = ch * CH, e = std::min(n, a + CH);
+        int *pos = &cc[8 * (size_t)ch];
+        for (int i = a; i < e; i++) tmp[pos[key[i]]++] = pts[i];
+    });
+    parallel_chunks(nch, [&](int ch, int) {
+        int a = ch * CH, e = std::min(n, a + CH);
+        std::copy(tmp.begin() + a, tmp.begin() + e, pts + a);
+    });
 }
 
 // Two boxes touch or overlap (src/d3tstrcr.f:1082-1179). The reference
@@ -136,6 +184,16 @@ inline bool touching(const int *ijk_a, int la, const int *ijk_b, int lb)
     }
     return true;
 }
+
+// Chunked list output: per-box counts plus the items of each chunk of
+// consecutive boxes in box order, assembled into CSR at the end.
+struct ChunkLists {
+    std::vector<int> cnt;                      // per box
+    std::vector<std::vector<int>> items;       // per chunk
+};
+
+constexpr int BOX_CHUNK = 64;
+constexpr int MAXCOLL = 26;
 
 }  // namespace
 
@@ -164,11 +222,18 @@ int build_tree(const double *src, int n, const double *targ, int nt, int nbox, i
         if (size < 2 * zm) size = 2 * zm;
     }
     T.size = size;
+    int nth = host_threads();
 
     // point records in tree order; permutations are read back at the end
     std::vector<Pt> ps(n), pt(nt);
-    for (int i = 0; i < n; i++) ps[i] = Pt{{src[3 * i], src[3 * i + 1], src[3 * i + 2]}, i};
-    for (int i = 0; i < nt; i++) pt[i] = Pt{{targ[3 * i], targ[3 * i + 1], targ[3 * i + 2]}, i};
+    parallel_chunks((n + 16383) / 16384, [&](int ch, int) {
+        for (int i = ch * 16384; i < std::min(n, (ch + 1) * 16384); i++)
+            ps[i] = Pt{{src[3 * i], src[3 * i + 1], src[3 * i + 2]}, i};
+    });
+    parallel_chunks((nt + 16383) / 16384, [&](int ch, int) {
+        for (int i = ch * 16384; i < std::min(nt, (ch + 1) * 16384); i++)
+            pt[i] = Pt{{targ[3 * i], targ[3 * i + 1], targ[3 * i + 2]}, i};
+    });
     T.level = {0};
     T.ijk = {1, 1, 1};
     T.parent = {-1};
@@ -181,44 +246,71 @@ int build_tree(const double *src, int n, const double *targ, int nt, int nbox, i
     T.laddr = {0, 1};
     T.nlev = 0;
 
-    std::vector<Pt> tmp;
+    // level-by-level subdivision (src/d3tstrcr.f:1497-1545)
+    std::vector<std::vector<Pt>> tmps(nth);
+    std::vector<unsigned char> keybuf;
     for (int lev = 0; lev < 100; lev++) {
         int first = T.laddr[2 * lev], count = T.laddr[2 * lev + 1];
-        int nnew = 0;
         double side = size / (double)(1 << lev), side2 = side / 2;
-        for (int b = first; b < first + count; b++) {
-            if (T.src_count[b] <= nbox && T.tgt_count[b] <= nbox) continue;
+        std::vector<int> sp;
+        for (int b = first; b < first + count; b++)
+            if (T.src_count[b] > nbox || T.tgt_count[b] > nbox) sp.push_back(b);
+        if (sp.empty()) break;
+        std::vector<int> cs(8 * sp.size()), ct(8 * sp.size());
+        auto centre = [&](int b, double *c) {
             // box centre of d3tcentf (src/d3tstrcr.f:1682-1705)
-            double c[3];
             for (int d = 0; d < 3; d++)
                 c[d] = T.center[d] - size / 2 + (T.ijk[3 * b + d] - 1) * side + side2;
-            int cs[8], ct[8];
-            split_octants(ps.data() + T.src_start[b], T.src_count[b], c, cs, tmp);
-            split_octants(pt.data() + T.tgt_start[b], T.tgt_count[b], c, ct, tmp);
+        };
+        if ((int)sp.size() >= 2 * nth) {
+            parallel_chunks((int)sp.size(), [&](int q, int tid) {
+                int b = sp[q];
+                double c[3];
+                centre(b, c);
+                split_octants(ps.data() + T.src_start[b], T.src_count[b], c, &cs[8 * q], tmps[tid]);
+                split_octants(pt.data() + T.tgt_start[b], T.tgt_count[b], c, &ct[8 * q], tmps[tid]);
+            });
+        } else {
+            for (size_t q = 0; q < sp.size(); q++) {
+                int b = sp[q];
+                double c[3];
+                centre(b, c);
+                auto split = [&](std::vector<Pt> &P, int s0, int cnum, int *out) {
+                    if (cnum >= 32768 && nth > 1)
+                        split_octants_par(P.data() + s0, cnum, c, out, tmps[0], keybuf);
+                    else
+                        split_octants(P.data() + s0, cnum, c, out, tmps[0]);
+                };
+                split(ps, T.src_start[b], T.src_count[b], &cs[8 * q]);
+                split(pt, T.tgt_start[b], T.tgt_count[b], &ct[8 * q]);
+            }
+        }
+        int nnew = 0;
+        for (size_t q = 0; q < sp.size(); q++) {
+            int b = sp[q];
             int s0 = T.src_start[b], t0 = T.tgt_start[b];
             for (int o = 0; o < 8; o++) {
-                int sa = s0, ta = t0;
-                s0 += cs[o];
-                t0 += ct[o];
-                if (cs[o] == 0 && ct[o] == 0) continue;   // empty octants make no box
+                int sa = s0, ta = t0, ns_ = cs[8 * q + o], nt_ = ct[8 * q + o];
+                s0 += ns_;
+                t0 += nt_;
+                if (ns_ == 0 && nt_ == 0) continue;   // empty octants make no box
                 int kid = (int)T.level.size();
                 T.level.push_back(lev + 1);
                 T.ijk.push_back(2 * (T.ijk[3 * b] - 1) + 1 + (o & 1));
                 T.ijk.push_back(2 * (T.ijk[3 * b + 1] - 1) + 1 + ((o >> 1) & 1));
                 T.ijk.push_back(2 * (T.ijk[3 * b + 2] - 1) + 1 + ((o >> 2) & 1));
                 T.parent.push_back(b);
-                for (int q = 0; q < 8; q++) T.kids.push_back(-1);
+                for (int k = 0; k < 8; k++) T.kids.push_back(-1);
                 T.nkids.push_back(0);
                 T.src_start.push_back(sa);
-                T.src_count.push_back(cs[o]);
+                T.src_count.push_back(ns_);
                 T.tgt_start.push_back(ta);
-                T.tgt_count.push_back(ct[o]);
+                T.tgt_count.push_back(nt_);
                 T.kids[8 * b + T.nkids[b]] = kid;   // compacted child slots
                 T.nkids[b]++;
                 nnew++;
             }
         }
-        if (nnew == 0) break;
         T.laddr.push_back(first + count);
         T.laddr.push_back(nnew);
         T.nlev = lev + 1;
@@ -238,84 +330,156 @@ int build_tree(const double *src, int n, const double *targ, int nt, int nbox, i
             T.centers[3 * b + d] = T.center[d] - size / 2 + (T.ijk[3 * b + d] - 1) * side + side2;
     }
     auto touch = [&](int a, int b) { return touching(&T.ijk[3 * a], T.level[a], &T.ijk[3 * b], T.level[b]); };
+    int nchunk = (nb + BOX_CHUNK - 1) / BOX_CHUNK;
 
     // lists 5 (colleagues) and 2 (well separated children of the parent's
-    // colleagues), src/d3tstrcr.f:760-828
-    std::vector<std::vector<int>> L[6];
-    for (int t = 1; t <= 5; t++) L[t].assign(nb, {});
-    for (int b = 0; b < nb; b++) {
-        L[5][b].reserve(26);
-        L[2][b].reserve(64);
-    }
-    for (int b = 1; b < nb; b++) {
-        int dad = T.parent[b];
-        int ncoll = 1 + (int)L[5][dad].size();
-        for (int ci = 0; ci < ncoll; ci++) {
-            int coll = ci == 0 ? dad : L[5][dad][ci - 1];
-            for (int q = 0; q < T.nkids[coll]; q++) {
-                int kid = T.kids[8 * coll + q];
-                if (kid == b) continue;
-                if (touch(kid, b))
-                    L[5][b].push_back(kid);
-                else
-                    L[2][b].push_back(kid);
+    // colleagues), src/d3tstrcr.f:760-828; level by level, since a box reads
+    // its parent's colleagues
+    std::vector<int> l5(MAXCOLL * (size_t)nb), n5(nb, 0);
+    ChunkLists L2{std::vector<int>(nb, 0), std::vector<std::vector<int>>(nchunk)};
+    for (int lev = 1; lev <= T.nlev; lev++) {
+        int f = T.laddr[2 * lev], e = f + T.laddr[2 * lev + 1];
+        int c0 = f / BOX_CHUNK, c1 = (e - 1) / BOX_CHUNK;
+        // chunks straddling a level boundary are split between two levels;
+        // each level writes its own boxes' part in box order
+        std::vector<std::vector<int>> part(c1 - c0 + 1);
+        parallel_chunks(c1 - c0 + 1, [&](int k, int) {
+            int a = std::max(f, (c0 + k) * BOX_CHUNK), z = std::min(e, (c0 + k + 1) * BOX_CHUNK);
+            std::vector<int> &out = part[k];
+            for (int b = a; b < z; b++) {
+                int dad = T.parent[b];
+                int ncoll = 1 + n5[dad];
+                int before = (int)out.size();
+                const int *ib = &T.ijk[3 * b];
+                for (int ci = 0; ci < ncoll; ci++) {
+                    int coll = ci == 0 ? dad : l5[MAXCOLL * (size_t)dad + ci - 1];
+                    for (int q = 0; q < T.nkids[coll]; q++) {
+                        int kid = T.kids[8 * coll + q];
+                        if (kid == b) continue;
+                        // same level: touching means every coordinate
+                        // differs by at most one
+                        const int *ik = &T.ijk[3 * kid];
+                        if (std::abs(ik[0] - ib[0]) <= 1 && std::abs(ik[1] - ib[1]) <= 1 &&
+                            std::abs(ik[2] - ib[2]) <= 1)
+                            l5[MAXCOLL * (size_t)b + n5[b]++] = kid;
+                        else
+                            out.push_back(kid);
+                    }
+                }
+                L2.cnt[b] = (int)out.size() - before;
             }
+        });
+        for (int k = 0; k <= c1 - c0; k++) {
+            auto &dst = L2.items[c0 + k];
+            dst.insert(dst.end(), part[k].begin(), part[k].end());
         }
     }
+
     // lists 1 and 3 for childless boxes: depth-first descent into each
     // colleague, children taken from the last compacted slot backwards
-    // (src/d3tstrcr.f:919-1076)
+    // (src/d3tstrcr.f:919-1076). A coarse leaf found touching a finer one
+    // is also appended to the finer leaf's list 1 (src/d3tstrcr.f:1021):
+    // those appends happen before the finer leaf's own entries (it has the
+    // larger index) and in increasing order of the coarse leaf, so they are
+    // collected as (fine, coarse) pairs and placed first.
+    ChunkLists L1{std::vector<int>(nb, 0), std::vector<std::vector<int>>(nchunk)};
+    ChunkLists L3{std::vector<int>(nb, 0), std::vector<std::vector<int>>(nchunk)};
+    std::vector<std::vector<int>> app(nchunk);    // (fine, coarse) pairs per chunk
     struct Frame {
         int box, left;
     };
-    std::vector<Frame> stack;
-    stack.reserve(128);
-    for (int b = 0; b < nb; b++) {
-        if (T.nkids[b] > 0 || T.level[b] == 0) continue;
-        for (int c0 : L[5][b]) {
-            stack.clear();
-            stack.push_back({c0, T.nkids[c0]});
-            while (!stack.empty()) {
-                Frame &f = stack.back();
-                int j = f.box;
-                if (!touch(b, j)) {
-                    L[3][b].push_back(j);
-                    stack.pop_back();
-                    stack.back().left--;
-                    continue;
+    parallel_chunks(nchunk, [&](int ch, int) {
+        std::vector<Frame> stack;
+        stack.reserve(128);
+        std::vector<int> &o1 = L1.items[ch], &o3 = L3.items[ch], &ap = app[ch];
+        for (int b = ch * BOX_CHUNK; b < std::min(nb, (ch + 1) * BOX_CHUNK); b++) {
+            if (T.nkids[b] > 0 || T.level[b] == 0) continue;
+            int b1 = (int)o1.size(), b3 = (int)o3.size();
+            for (int k = 0; k < n5[b]; k++) {
+                int c0 = l5[MAXCOLL * (size_t)b + k];
+                stack.clear();
+                stack.push_back({c0, T.nkids[c0]});
+                while (!stack.empty()) {
+                    Frame &fr = stack.back();
+                    int j = fr.box;
+                    if (!touch(b, j)) {
+                        o3.push_back(j);
+                        stack.pop_back();
+                        stack.back().left--;
+                        continue;
+                    }
+                    if (T.nkids[j] == 0) {
+                        o1.push_back(j);
+                        if (T.level[j] != T.level[b]) {
+                            ap.push_back(j);
+                            ap.push_back(b);
+                        }
+                        stack.pop_back();
+                        if (!stack.empty()) stack.back().left--;
+                        continue;
+                    }
+                    if (fr.left == 0) {
+                        stack.pop_back();
+                        if (!stack.empty()) stack.back().left--;
+                        continue;
+                    }
+                    int kid = T.kids[8 * j + fr.left - 1];
+                    stack.push_back({kid, T.nkids[kid]});
                 }
-                if (T.nkids[j] == 0) {
-                    L[1][b].push_back(j);
-                    if (T.level[j] != T.level[b]) L[1][j].push_back(b);
-                    stack.pop_back();
-                    if (!stack.empty()) stack.back().left--;
-                    continue;
-                }
-                if (f.left == 0) {
-                    stack.pop_back();
-                    if (!stack.empty()) stack.back().left--;
-                    continue;
-                }
-                int kid = T.kids[8 * j + f.left - 1];
-                stack.push_back({kid, T.nkids[kid]});
             }
+            L1.cnt[b] = (int)o1.size() - b1;
+            L3.cnt[b] = (int)o3.size() - b3;
         }
-    }
-    // list 4 is the dual of list 3
-    for (int b = 0; b < nb; b++)
-        for (int j : L[3][b]) L[4][j].push_back(b);
+    });
 
-    for (int t = 1; t <= 5; t++) {
+    auto to_csr = [&](ChunkLists &L, int t) {
         T.lstart[t].assign(nb + 1, 0);
-        size_t tot = 0;
-        for (int b = 0; b < nb; b++) tot += L[t][b].size();
-        T.litems[t].resize(tot);
-        size_t pos = 0;
-        for (int b = 0; b < nb; b++) {
-            T.lstart[t][b] = (int)pos;
-            for (int j : L[t][b]) T.litems[t][pos++] = j;
-        }
-        T.lstart[t][nb] = (int)pos;
+        for (int b = 0; b < nb; b++) T.lstart[t][b + 1] = T.lstart[t][b] + L.cnt[b];
+        T.litems[t].resize(T.lstart[t][nb]);
+        parallel_chunks(nchunk, [&](int ch, int) {
+            int b0 = ch * BOX_CHUNK;
+            if (b0 >= nb) return;
+            std::copy(L.items[ch].begin(), L.items[ch].end(), T.litems[t].begin() + T.lstart[t][b0]);
+        });
+    };
+    to_csr(L2, 2);
+    to_csr(L3, 3);
+    // list 5 from the flat colleague table
+    T.lstart[5].assign(nb + 1, 0);
+    for (int b = 0; b < nb; b++) T.lstart[5][b + 1] = T.lstart[5][b] + n5[b];
+    T.litems[5].resize(T.lstart[5][nb]);
+    for (int b = 0; b < nb; b++)
+        std::copy(&l5[MAXCOLL * (size_t)b], &l5[MAXCOLL * (size_t)b] + n5[b], T.litems[5].begin() + T.lstart[5][b]);
+    // list 1: appended coarse leaves first, then the box's own entries
+    {
+        std::vector<int> acnt(nb, 0);
+        for (auto &ap : app)
+            for (size_t k = 0; k < ap.size(); k += 2) acnt[ap[k]]++;
+        T.lstart[1].assign(nb + 1, 0);
+        for (int b = 0; b < nb; b++) T.lstart[1][b + 1] = T.lstart[1][b] + acnt[b] + L1.cnt[b];
+        T.litems[1].resize(T.lstart[1][nb]);
+        std::vector<int> fill(nb);
+        for (int b = 0; b < nb; b++) fill[b] = T.lstart[1][b];
+        for (auto &ap : app)
+            for (size_t k = 0; k < ap.size(); k += 2) T.litems[1][fill[ap[k]]++] = ap[k + 1];
+        parallel_chunks(nchunk, [&](int ch, int) {
+            size_t pos = 0;
+            for (int b = ch * BOX_CHUNK; b < std::min(nb, (ch + 1) * BOX_CHUNK); b++) {
+                std::copy(L1.items[ch].begin() + pos, L1.items[ch].begin() + pos + L1.cnt[b],
+                          T.litems[1].begin() + fill[b]);
+                pos += L1.cnt[b];
+            }
+        });
+    }
+    // list 4 is the dual of list 3, in increasing box order
+    {
+        T.lstart[4].assign(nb + 1, 0);
+        for (int j : T.litems[3]) T.lstart[4][j + 1]++;
+        for (int b = 0; b < nb; b++) T.lstart[4][b + 1] += T.lstart[4][b];
+        T.litems[4].resize(T.lstart[4][nb]);
+        std::vector<int> fill(T.lstart[4].begin(), T.lstart[4].end() - 1);
+        for (int b = 0; b < nb; b++)
+            for (int q = T.lstart[3][b]; q < T.lstart[3][b + 1]; q++) T.litems[4][fill[T.litems[3][q]]++] = b;
     }
     return 0;
 }

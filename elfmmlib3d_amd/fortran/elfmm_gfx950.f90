@@ -71,10 +71,10 @@ module elfmm_gfx950
        type(c_ptr), value :: stream
      end function elfmm_gfx950_triatarg_dev
 
-     ! Phase times (ms) of the last call on this thread, 12 entries.
-     subroutine elfmm_gfx950_last_timings(ms12) bind(C, name="elfmm_gfx950_last_timings")
+     ! Phase times (ms) of the last call on this thread, 16 entries.
+     subroutine elfmm_gfx950_last_timings(ms16) bind(C, name="elfmm_gfx950_last_timings")
        import :: c_double
-       real(c_double) :: ms12(12)
+       real(c_double) :: ms16(16)
      end subroutine elfmm_gfx950_last_timings
 
      ! Work counts of the last call on this thread, 16 entries.

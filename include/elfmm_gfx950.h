@@ -20,7 +20,7 @@
  *
  * Error codes (ier): 0 ok; 4 tree construction failed; 8 device memory
  * allocation failed; 16 unsupported precision flag on this device path
- * (iprec > 3); 32 HIP runtime error. Errors never abort the process.
+ * (iprec outside -2..3); 32 HIP runtime error. Errors never abort the process.
  */
 #ifndef ELFMM_GFX950_H
 #define ELFMM_GFX950_H
@@ -72,9 +72,11 @@ int elfmm_gfx950_triatarg_dev(int iprec, double rlam, double rmu, int nparts,
 /* Per-phase times (milliseconds) of the last call on this thread:
  * [0] tree+plan (host), [1] upload, [2] P2M, [3] M2M, [4] M2L, [5] P2L,
  * [6] L2L, [7] L2T, [8] M2T, [9] near field, [10] assembly, [11] total
- * wall time of the call. [0] is host time; [1]-[10] are HIP-event times on
- * the stream each phase runs on (the near field overlaps [2]-[8]). */
-void elfmm_gfx950_last_timings(double *ms12);
+ * wall time of the call, [12] octree and lists (part of [0]), [13]
+ * interaction plan (part of [0]), [14]-[15] reserved. [0], [12], [13] are
+ * host times; [1]-[10] are HIP-event times on the stream each phase runs
+ * on (the near field overlaps [2]-[8]). */
+void elfmm_gfx950_last_timings(double *ms16);
 
 /* Work counts of the last call on this thread: [0] boxes, [1] levels,
  * [2] expansion order p, [3] M2L pairs, [4] M2L flops as executed (dense

@@ -1,0 +1,48 @@
+// C wrapper around the library's host octree (elfmm_tree.cpp) so the tests
+// can compare it with the reference's d3tstrcr without a GPU.
+#include <cstring>
+
+#include "../../elfmmlib3d_amd/csrc/elfmm_internal.h"
+
+static elfmm::Tree g_tree;
+
+extern "C" int tree_build(const double *src, int n, const double *targ, int nt, int nbox, int imindlin)
+{
+    g_tree = elfmm::Tree();
+    if (elfmm::build_tree(src, n, targ, nt, nbox, imindlin, g_tree)) return -1;
+    return g_tree.nboxes;
+}
+
+// sizes: [0] nboxes, [1] nlev, [2..6] total items of lists 1..5
+extern "C" void tree_sizes(int *out)
+{
+    out[0] = g_tree.nboxes;
+    out[1] = g_tree.nlev;
+    for (int t = 1; t <= 5; t++) out[1 + t] = (int)g_tree.litems[t].size();
+}
+
+// boxes[nb][7]: level, i, j, k (1-based), parent (0-based, -1 root),
+// nkids, src_count; centers[nb][3]; iz[n]; iztarg[nt]; lstart/litems per list
+extern "C" void tree_get(int *boxes, double *centers, int *iz, int *iztarg, int *lstart, int *litems)
+{
+    const elfmm::Tree &T = g_tree;
+    for (int b = 0; b < T.nboxes; b++) {
+        int *o = boxes + 7 * b;
+        o[0] = T.level[b];
+        o[1] = T.ijk[3 * b];
+        o[2] = T.ijk[3 * b + 1];
+        o[3] = T.ijk[3 * b + 2];
+        o[4] = T.parent[b];
+        o[5] = T.nkids[b];
+        o[6] = T.src_count[b];
+    }
+    std::memcpy(centers, T.centers.data(), 3 * sizeof(double) * T.nboxes);
+    if (!T.iz.empty()) std::memcpy(iz, T.iz.data(), sizeof(int) * T.iz.size());
+    if (!T.iztarg.empty()) std::memcpy(iztarg, T.iztarg.data(), sizeof(int) * T.iztarg.size());
+    for (int t = 1; t <= 5; t++) {
+        std::memcpy(lstart, T.lstart[t].data(), sizeof(int) * (T.nboxes + 1));
+        lstart += T.nboxes + 1;
+        if (!T.litems[t].empty()) std::memcpy(litems, T.litems[t].data(), sizeof(int) * T.litems[t].size());
+        litems += T.litems[t].size();
+    }
+}

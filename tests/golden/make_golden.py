@@ -1,7 +1,8 @@
 """Write the golden fixtures from the reference compiled from its sources.
 
 Run in a container that has the reference tree (make -C oracle ref first):
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
+(no arguments: every case).
 Each fixture holds its inputs and the reference's outputs, so the tests
 never need the reference itself. Cases (all FP64):
   fmm_s6        icosphere 6x6/face (720 tris) + 300 shell targets, iprec 2,
@@ -12,6 +13,10 @@ never need the reference itself. Cases (all FP64):
   fmm_sl_only   fmm_s6 geometry, single layer only, targets only outputs
   fmm_dl_only   fmm_s6 geometry, double layer only, centroid outputs only
   fmm_tiny      one-level tree (40 tris, all near field)
+  fmm_s6_p3     fmm_s6 geometry at iprec 3 (p = 39)
+  fmm_s6_m1     fmm_s6 geometry at iprec -1
+  fmm_fault     4 planar faults (~3000 right triangles, open surfaces) with
+                1000 targets around them, lam=1, mu=0.8, iprec 2
   pairs         500 random (target, triangle) pairs incl. self and
                 near-coplanar targets: elust3/eltst3triadirecttarg_one
   tables        random plane-triangle tables (triabtable / triartable)
@@ -58,16 +63,43 @@ def fmm_case(name, iprec, lam, mu, tri, nrm, src, ssl, sdl, tg, flags=(1, 1, 1, 
          target=tg, flags=np.array(flags), ptfrc=pt, strain=st, ptfrctarg=ptt, straintarg=stt)
 
 
-def main():
+def fault_case(seed=13, ntri=3000, ntarg=1000):
+    rng = np.random.default_rng(seed)
+    tri = mesh.fault_mesh(4, ntri, rng, box=8.0)
+    nrm = mesh.triangle_normals(tri)
+    src = mesh.centroids(tri)
+    ssl = rng.uniform(-1, 1, (tri.shape[0], 3))
+    sdl = rng.uniform(-1, 1, (tri.shape[0], 3))
+    lo, hi = tri.reshape(-1, 3).min(0), tri.reshape(-1, 3).max(0)
+    tg = rng.uniform(lo - 0.5, hi + 0.5, (ntarg, 3))
+    return tri, nrm, src, ssl, sdl, tg
+
+
+def main(only=None):
+    want = lambda name: only is None or name in only
     tri, nrm, src, ssl, sdl, tg = sphere_case(6, 300, 11)
-    fmm_case("fmm_s6", 2, 1.3, 0.9, tri, nrm, src, ssl, sdl, tg)
-    fmm_case("fmm_sl_only", 2, 1.3, 0.9, tri, nrm, src, ssl, None, tg, (0, 0, 1, 1))
-    fmm_case("fmm_dl_only", 2, 1.3, 0.9, tri, nrm, src, None, sdl, np.zeros((0, 3)), (1, 1, 0, 0))
+    if want("fmm_s6"):
+        fmm_case("fmm_s6", 2, 1.3, 0.9, tri, nrm, src, ssl, sdl, tg)
+    if want("fmm_sl_only"):
+        fmm_case("fmm_sl_only", 2, 1.3, 0.9, tri, nrm, src, ssl, None, tg, (0, 0, 1, 1))
+    if want("fmm_dl_only"):
+        fmm_case("fmm_dl_only", 2, 1.3, 0.9, tri, nrm, src, None, sdl, np.zeros((0, 3)), (1, 1, 0, 0))
+    if want("fmm_s6_p3"):
+        fmm_case("fmm_s6_p3", 3, 1.3, 0.9, tri, nrm, src, ssl, sdl, tg)
+    if want("fmm_s6_m1"):
+        fmm_case("fmm_s6_m1", -1, 1.3, 0.9, tri, nrm, src, ssl, sdl, tg)
     tri, nrm, src, ssl, sdl, tg = sphere_case(12, 1000, 1)
-    fmm_case("fmm_c1", 2, 1.0, 1.0, tri, nrm, src, ssl, sdl, tg)
-    fmm_case("fmm_c1_p1", 1, 1.0, 1.0, tri, nrm, src, ssl, sdl, tg)
-    t2, n2, s2, sl2, dl2, tg2 = sphere_case(2, 10, 5)
-    fmm_case("fmm_tiny", 2, 2.0, 0.7, t2[:40], n2[:40], s2[:40], sl2[:40], dl2[:40], tg2)
+    if want("fmm_c1"):
+        fmm_case("fmm_c1", 2, 1.0, 1.0, tri, nrm, src, ssl, sdl, tg)
+    if want("fmm_c1_p1"):
+        fmm_case("fmm_c1_p1", 1, 1.0, 1.0, tri, nrm, src, ssl, sdl, tg)
+    if want("fmm_tiny"):
+        t2, n2, s2, sl2, dl2, tg2 = sphere_case(2, 10, 5)
+        fmm_case("fmm_tiny", 2, 2.0, 0.7, t2[:40], n2[:40], s2[:40], sl2[:40], dl2[:40], tg2)
+    if want("fmm_fault"):
+        fmm_case("fmm_fault", 2, 1.0, 0.8, *fault_case())
+    if only is not None:
+        return
 
     # pair quadratures
     rng = np.random.default_rng(7)
@@ -148,4 +180,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:] or None)

@@ -21,7 +21,8 @@ def run_golden(gpu_lib, g):
                                    g["trinorm"], g["source"], ssl, sdl, g["target"], *f)
 
 
-@pytest.mark.parametrize("case", ["fmm_tiny", "fmm_s6", "fmm_c1", "fmm_c1_p1", "fmm_sl_only", "fmm_dl_only"])
+@pytest.mark.parametrize("case", ["fmm_tiny", "fmm_s6", "fmm_c1", "fmm_c1_p1", "fmm_sl_only", "fmm_dl_only",
+                                  "fmm_s6_p3", "fmm_s6_m1", "fmm_fault"])
 def test_fmm_matches_reference_fmm(gpu_lib, case):
     g = load_golden(case)
     print(f"case {case}", flush=True)

@@ -1,0 +1,97 @@
+"""The library's host octree (elfmm_tree.cpp, parallel over the host pool)
+against the reference's d3tstrcr: bit-exact boxes, point permutations and
+list order on the reference fixture, and against the oracle's restatement
+(itself pinned to the reference) on inputs chosen to stress the rules:
+ties on box centres, deep trees, the half-space root, no targets, and
+boxes large enough for the multi-threaded split."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, load_golden
+from elfmmlib3d_amd import mesh
+
+CSRC = os.path.join(ROOT, "elfmmlib3d_amd", "csrc")
+
+
+@pytest.fixture(scope="module")
+def tree_lib(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("tree") / "libtree_check.so")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-o", out,
+                    os.path.join(ROOT, "tests", "csrc", "tree_check.cpp"),
+                    os.path.join(CSRC, "elfmm_tree.cpp"), os.path.join(CSRC, "elfmm_pool.cpp")],
+                   check=True)
+    return C.CDLL(out)
+
+
+def lib_tree(lib, src, tgt, nbox, imindlin=0):
+    n, m = len(src), len(tgt)
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    tgt = np.ascontiguousarray(tgt if m else np.zeros((1, 3)), dtype=np.float64)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    nb = lib.tree_build(p(src), n, p(tgt), m, nbox, imindlin)
+    assert nb > 0
+    sz = np.zeros(7, dtype=np.int32)
+    lib.tree_sizes(p(sz))
+    boxes = np.zeros((nb, 7), dtype=np.int32)
+    centers = np.zeros((nb, 3))
+    iz = np.zeros(n, dtype=np.int32)
+    izt = np.zeros(max(m, 1), dtype=np.int32)
+    lstart = np.zeros(5 * (nb + 1), dtype=np.int32)
+    litems = np.zeros(max(int(sz[2:].sum()), 1), dtype=np.int32)
+    lib.tree_get(p(boxes), p(centers), p(iz), p(izt), p(lstart), p(litems))
+    lists, off = {}, 0
+    for t in range(1, 6):
+        st = lstart[(t - 1) * (nb + 1):t * (nb + 1)]
+        lists[t] = (st.copy(), litems[off:off + st[-1]].copy())
+        off += st[-1]
+    return dict(boxes=boxes, centers=centers, iz=iz, iztarg=izt[:m], nlev=int(sz[1]), lists=lists)
+
+
+def compare_to_reference_layout(t, boxes, centers, iz, iztarg, nlev, lstart, litems):
+    """boxes/lists in the reference's 1-based box(20) layout."""
+    assert t["boxes"].shape[0] == boxes.shape[0]
+    assert t["nlev"] == nlev
+    assert np.array_equal(t["boxes"][:, 0:4], boxes[:, 0:4])                  # level, i, j, k
+    assert np.array_equal(t["boxes"][:, 4] + 1, boxes[:, 4])                   # parent
+    assert np.array_equal(t["boxes"][:, 6], boxes[:, 14])                      # source count
+    assert np.array_equal(t["centers"], centers)
+    assert np.array_equal(t["iz"] + 1, iz)
+    assert np.array_equal(t["iztarg"] + 1, iztarg)
+    for typ in range(1, 6):
+        st, it = t["lists"][typ]
+        assert np.array_equal(st, lstart[typ]), typ
+        assert np.array_equal(it + 1, litems[typ]), typ
+
+
+def test_tree_matches_reference_fixture(tree_lib):
+    g = load_golden("tree_c1")
+    t = lib_tree(tree_lib, g["source"], g["target"], int(g["nbox"]))
+    compare_to_reference_layout(t, g["boxes"], g["centers"], g["iz"], g["iztarg"], int(g["nlev"]),
+                                {k: g[f"lstart{k}"] for k in range(1, 6)},
+                                {k: g[f"litems{k}"] for k in range(1, 6)})
+
+
+def cases():
+    rng = np.random.default_rng(3)
+    yield "uniform", rng.uniform(-1, 1, (3000, 3)), rng.uniform(-1, 1, (500, 3)), 20, 0
+    # coordinates on a coarse integer grid: many ties with box centres
+    yield "ties", rng.integers(-4, 5, (2000, 3)).astype(float), rng.integers(-4, 5, (300, 3)).astype(float), 10, 0
+    yield "deep", mesh.centroids(mesh.icosphere(6)), np.zeros((0, 3)), 2, 0
+    tri = mesh.fault_mesh(3, 2000, rng, box=6.0)
+    yield "halfspace", mesh.centroids(tri), rng.uniform(-3, 3, (400, 3)) - [0, 0, 3.5], 40, 1
+    # more than 32768 points of each kind: the root split runs on all pool threads
+    yield "large", mesh.centroids(mesh.icosphere(42)), rng.uniform(-2, 2, (40000, 3)), 40, 0
+
+
+@pytest.mark.parametrize("case", list(range(5)))
+def test_tree_matches_oracle(tree_lib, oracle_lib, case):
+    name, src, tgt, nbox, imindlin = list(cases())[case]
+    t = lib_tree(tree_lib, src, tgt, nbox, imindlin)
+    o = oracle_lib.tree(src, tgt, nbox, imindlin)
+    lst = {k: np.cumsum([0] + [len(x) for x in o["lists"][k]]).astype(np.int32) for k in range(1, 6)}
+    itm = {k: (np.concatenate(o["lists"][k]) if lst[k][-1] else np.zeros(0, np.int32)) for k in range(1, 6)}
+    compare_to_reference_layout(t, o["boxes"], o["centers"], o["iz"], o["iztarg"], o["nlev"], lst, itm)

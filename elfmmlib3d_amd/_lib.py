@@ -114,6 +114,45 @@ def el3dtriadirecttarg(rlam, rmu, triangle, trinorm, source, sigma_sl=None, sigm
     return pt, st, ptt[:m], stt[:m]
 
 
+def elfmm3dtriatargiter(iprec, rlam, rmu, triangle, trinorm, source, sigma_sl=None, sigma_dl=None,
+                        target=None, ifptfrc=1, ifstrain=1, ifptfrctarg=1, ifstraintarg=1,
+                        icomp_type=1, wsave=None, ptfrc=None, strain=None, ptfrctarg=None,
+                        straintarg=None):
+    """Iter FMM (src/elfmm3dtria_iter_1.f:160): icomp_type 1 compute, 2 compute
+    and keep the near-field interactions, 3 replay them with new densities,
+    4 plan only (returns lused), 5 store and add the near field only. The
+    stored interactions live in device memory; wsave (float64, at least the
+    lused words icomp 4 returns) carries their handle between calls.
+    Output arrays may be passed in (icomp 5 adds to them). Returns
+    (ier, lused, ptfrc, strain, ptfrctarg, straintarg)."""
+    n, tri, nrm, src, ifs, ssl, ifd, sdl, m, tgt = _prep(triangle, trinorm, source, sigma_sl,
+                                                          sigma_dl, target)
+    pt = np.zeros((n, 3)) if ptfrc is None else np.ascontiguousarray(ptfrc, dtype=np.float64)
+    st = np.zeros((n, 3, 3)) if strain is None else np.ascontiguousarray(strain, dtype=np.float64)
+    ptt = np.zeros((max(m, 1), 3))
+    stt = np.zeros((max(m, 1), 3, 3))
+    if ptfrctarg is not None and m:
+        ptt[:m] = ptfrctarg
+    if straintarg is not None and m:
+        stt[:m] = straintarg
+    if wsave is None:
+        wsave = np.zeros(1)
+    ier, lused = C.c_int(0), C.c_int(0)
+    library().elfmm3dtriatargiter_(C.byref(ier), _iref(iprec), _dref(rlam), _dref(rmu), _p(tri), _p(nrm),
+                                   _iref(n), _p(src), _iref(ifs), _p(ssl), _iref(ifd), _p(sdl),
+                                   _iref(ifptfrc), _p(pt), _iref(ifstrain), _p(st), _iref(m), _p(tgt),
+                                   _iref(ifptfrctarg if m else 0), _p(ptt),
+                                   _iref(ifstraintarg if m else 0), _p(stt), _iref(icomp_type),
+                                   _p(wsave), _iref(wsave.size), C.byref(lused))
+    return ier.value, lused.value, pt, st, ptt[:m], stt[:m]
+
+
+def iter_release(wsave):
+    """Free the device-resident interactions named by wsave. Returns 1 if
+    a store was freed."""
+    return library().elfmm_gfx950_iter_release(_p(wsave), C.c_int(wsave.size))
+
+
 def elfmm3dtriatarg_dev(iprec, rlam, rmu, triangle, trinorm, source, sigma_sl, sigma_dl, target,
                         ptfrc, strain, ptfrctarg, straintarg, stream=None):
     """Device-resident apply on torch CUDA/HIP tensors (float64, contiguous).
@@ -136,7 +175,8 @@ def last_stats():
     out = np.zeros(16)
     library().elfmm_gfx950_last_stats(_p(out))
     keys = ["nboxes", "nlev", "p", "m2l_pairs", "m2l_flops_dense", "m2l_flops_alg", "near_pairs",
-            "m2l_tiles", "m2m_pairs", "l2l_pairs", "p2m_boxes", "p2l_pairs", "m2t_pairs", "l2t_boxes"]
+            "m2l_tiles", "m2m_pairs", "l2l_pairs", "p2m_boxes", "p2l_pairs", "m2t_pairs", "l2t_boxes",
+            "iter_ref_store_words", "iter_store_words"]
     return dict(zip(keys, out.tolist()))
 
 

@@ -17,6 +17,9 @@ void parallel_chunks(int nchunks, const std::function<void(int, int)> &fn);
 // lfmm3dtrilhesstarg (src/lfmm3dtrilhess.f:216-235): nbox = (120*2^k)/3.
 double iprec_eps(int iprec);
 int iprec_nbox(int iprec, int npts);
+// leaf size of the iter engine (src/lfmm3dtrilhess_iter_1.f:246-255): twice
+// the table above
+int iprec_nbox_iter(int iprec, int npts);
 // expansion order of l3dterms (src/l3dterms.f:42-85)
 int expansion_order(double eps);
 // list-2 truncation table of l3dterms_list2 (src/l3dterms.f:144-256),
@@ -60,5 +63,47 @@ struct Operators {
     std::vector<double> m2l;
 };
 void build_operators(int p, const int *itable, Operators &ops);
+
+// One tile of a dense translation launch: GT_M output rows of one operator
+// matrix applied to up to GT_P (source, destination) pairs.
+struct GemmTile {
+    int64_t a_off;   // matrix offset (doubles), row-major dim x dim
+    int dim;         // rows = cols
+    int row0;        // first output row of this tile
+    int pair0;       // first pair of this tile
+    int npair;       // <= GT_P
+};
+constexpr int GT_M = 64, GT_P = 16, GT_K = 16;
+
+struct GemmLaunch {
+    std::vector<GemmTile> tiles;
+    std::vector<int> src, dst;
+    std::vector<double> scale;
+    size_t tile_off = 0, off_src = 0, off_dst = 0, off_sc = 0;   // offsets in the device plan buffer
+};
+
+// Task lists of one apply, all box ids 0-based in tree order; points are
+// numbered sources first (0..n-1), then targets (n..n+nt-1).
+struct Plan {
+    std::vector<double> sides;                       // per box
+    std::vector<int> p2m_boxes;                      // leaves forming a multipole
+    std::vector<GemmLaunch> m2m_l, l2l_l;            // one launch per level
+    GemmLaunch m2l;                                  // all list-2 pairs, grouped by offset
+    std::vector<int> p2l_boxes, p2l_start, p2l_items;   // box <- sources of its list-4 boxes
+    std::vector<int> m2t_boxes, m2t_start, m2t_items;   // box <- multipoles of its list-3 boxes
+    std::vector<int> l2t_boxes, nf_boxes, nf_start, nf_items;
+    std::vector<int> l2t_tpt, l2t_tslot;             // (point, l2t slot) tasks
+    std::vector<int> m2t_slot_box, m2t_tpt, m2t_tslot;   // (point, m2t slot) tasks
+    double stats[16];                                // as elfmm_gfx950_last_stats
+};
+
+// Interaction plan of one apply (the work of the reference's
+// lfmm3dtrilhesstargmain steps 1-7 and of the near-field pass, as task
+// lists). ops == nullptr plans the direct sum only; nrhs is the number of
+// right-hand sides the work counts assume. iter selects the iter engine's
+// split: no P2L/M2T, and lists 3 and 4 join self + list 1 in the near
+// field (src/elfmm3dtria_iter_1.f:1061-1110).
+void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool direct_only, int nrhs,
+                Plan &P, bool iter = false);
 
 }  // namespace elfmm

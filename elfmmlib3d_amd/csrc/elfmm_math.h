@@ -333,93 +333,82 @@ ELFMM_HD int nf_i4(int i, int j, int k, int l)
     return start[nx] + (rest - ny);
 }
 
-// One (target, triangle) pair: accumulate displacement (3) and strain
-// (xx yy zz xy xz yz) in global coordinates into out[9]; c1 =
-// (lam+mu)/(lam+2mu); ifsl/ifdl select the layers. Two passes over the
-// edges keep the live state small: orders <= 3 (single layer, double-layer
-// displacement), then order 4 (double-layer strain).
-ELFMM_HD void nf_pair(const double *rec, const double *P, int iself, double rlam, double rmu,
-                      double c1, int ifsl, int ifdl, double *out)
+// Assembly of the elastic fields from the integral tables, in the
+// triangle's local frame (z along the normal), for local densities sg.
+// Single layer, displacement and strain (src/elfmm3dtria.f:1762-1840).
+ELFMM_HD void nf_sl_low(const TablesLow &T, const double *sg, double c1, double *disp, double *H)
 {
-    const double *U = rec;
-    double q0 = P[0] - rec[9], q1 = P[1] - rec[10], q2 = P[2] - rec[11];
-    double x0 = U[0] * q0 + U[1] * q1 + U[2] * q2;
-    double y0 = U[3] * q0 + U[4] * q1 + U[5] * q2;
-    double z0 = U[6] * q0 + U[7] * q1 + U[8] * q2;
-    double disp[3] = {0, 0, 0};
-    double H[6] = {0, 0, 0, 0, 0, 0};   // local xx xy xz yy yz zz
-    {
-        TablesLow T;
-        nf_tables_low(rec[12], rec[13], rec[14], x0, y0, z0, iself, T);
-        if (ifsl) {
-            const double *sg = rec + 15;
-            // disp_i = -c1 sum_k s_k b(i,k) + 2 s_i F
+    // disp_i = -c1 sum_k s_k b(i,k) + 2 s_i F
 #pragma unroll
-            for (int i = 0; i < 3; i++) {
-                double acc = 0.0;
+    for (int i = 0; i < 3; i++) {
+        double acc = 0.0;
 #pragma unroll
-                for (int k = 0; k < 3; k++) acc += sg[k] * T.b2[nf_i2(i, k)];
-                disp[i] += -c1 * acc + 2.0 * sg[i] * T.F;
-            }
-            // strain_ij = -[ -c1 sum_k s_k b(i,j,k) + s_i r(j) + s_j r(i) ]
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = i; j < 3; j++) {
-                    double acc = 0.0;
-#pragma unroll
-                    for (int k = 0; k < 3; k++) acc += sg[k] * T.b3[nf_i3(i, j, k)];
-                    H[nf_i2(i, j)] -= -c1 * acc + sg[i] * T.r1[j] + sg[j] * T.r1[i];
-                }
-        }
-        if (ifdl) {
-            const double *sg = rec + 18;
-            // Reference assembly (src/elfmm3dtria.f:2597-2634): with
-            // u(a,k,b) = -c1 b(a,b,k) + 2 d_ak r(b) and
-            // T(a,k) = lam d_k3 sum_l u(l,a,l) + mu (u(k,a,3) + u(3,a,k)),
-            // disp_a = sum_k s_k T(a,k). Summing over k first:
-            // disp_a = lam s_z (2 r(a) - c1 sum_l b(l,l,a))
-            //        + mu (-2 c1 sum_k s_k b(a,k,z) + 2 s_a r(z) + 2 d_az (s.r))
-            double sr = sg[0] * T.r1[0] + sg[1] * T.r1[1] + sg[2] * T.r1[2];
-#pragma unroll
-            for (int a = 0; a < 3; a++) {
-                double lap = T.b3[nf_i3(0, 0, a)] + T.b3[nf_i3(1, 1, a)] + T.b3[nf_i3(2, 2, a)];
-                double q = sg[0] * T.b3[nf_i3(a, 0, 2)] + sg[1] * T.b3[nf_i3(a, 1, 2)] + sg[2] * T.b3[nf_i3(a, 2, 2)];
-                disp[a] += rlam * sg[2] * (2.0 * T.r1[a] - c1 * lap) +
-                           rmu * (-2.0 * c1 * q + 2.0 * sg[a] * T.r1[2] + (a == 2 ? 2.0 * sr : 0.0));
-            }
-        }
+        for (int k = 0; k < 3; k++) acc += sg[k] * T.b2[nf_i2(i, k)];
+        disp[i] += -c1 * acc + 2.0 * sg[i] * T.F;
     }
-    if (ifdl) {
-        TablesHigh T;
-        nf_tables_high(rec[12], rec[13], rec[14], x0, y0, z0, iself, T);
-        const double *sg = rec + 18;
-        // Reference assembly (src/elfmm3dtria.f:2636-2660, 2692-2714): with
-        // d(a,k,b,m) = -c1 b(a,b,k,m) + 2 d_ak r(b,m) and
-        // S(a,k,b) = lam d_k3 sum_l d(l,a,l,b) + mu (d(k,a,3,b) + d(3,a,k,b)),
-        // strain_ab = -sym_ab sum_k s_k S(a,k,b). Summing over k first:
-        // lam s_z (2 r(a,b) - c1 sum_l b(l,l,a,b))
-        // + mu (-2 c1 sum_k s_k b(a,b,k,z) + s_a r(z,b) + s_b r(z,a)
-        //       + d_az (s.r)_b + d_bz (s.r)_a)
-        double sr[3];
+    // strain_ij = -[ -c1 sum_k s_k b(i,j,k) + s_i r(j) + s_j r(i) ]
 #pragma unroll
-        for (int b = 0; b < 3; b++)
-            sr[b] = sg[0] * T.r2[nf_i2(0, b)] + sg[1] * T.r2[nf_i2(1, b)] + sg[2] * T.r2[nf_i2(2, b)];
+    for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int a = 0; a < 3; a++)
+        for (int j = i; j < 3; j++) {
+            double acc = 0.0;
 #pragma unroll
-            for (int b = a; b < 3; b++) {
-                double lap = T.b4[nf_i4(0, 0, a, b)] + T.b4[nf_i4(1, 1, a, b)] + T.b4[nf_i4(2, 2, a, b)];
-                double q = sg[0] * T.b4[nf_i4(a, b, 0, 2)] + sg[1] * T.b4[nf_i4(a, b, 1, 2)] +
-                           sg[2] * T.b4[nf_i4(a, b, 2, 2)];
-                double v = rlam * sg[2] * (2.0 * T.r2[nf_i2(a, b)] - c1 * lap) +
-                           rmu * (-2.0 * c1 * q + sg[a] * T.r2[nf_i2(2, b)] + sg[b] * T.r2[nf_i2(2, a)] +
-                                  (a == 2 ? sr[b] : 0.0) + (b == 2 ? sr[a] : 0.0));
-                H[nf_i2(a, b)] -= v;
-            }
+            for (int k = 0; k < 3; k++) acc += sg[k] * T.b3[nf_i3(i, j, k)];
+            H[nf_i2(i, j)] -= -c1 * acc + sg[i] * T.r1[j] + sg[j] * T.r1[i];
+        }
+}
+
+// Double layer, displacement. Reference assembly (src/elfmm3dtria.f:
+// 2597-2634): with u(a,k,b) = -c1 b(a,b,k) + 2 d_ak r(b) and
+// T(a,k) = lam d_k3 sum_l u(l,a,l) + mu (u(k,a,3) + u(3,a,k)),
+// disp_a = sum_k s_k T(a,k). Summing over k first:
+// disp_a = lam s_z (2 r(a) - c1 sum_l b(l,l,a))
+//        + mu (-2 c1 sum_k s_k b(a,k,z) + 2 s_a r(z) + 2 d_az (s.r))
+ELFMM_HD void nf_dl_low(const TablesLow &T, const double *sg, double c1, double rlam, double rmu,
+                        double *disp)
+{
+    double sr = sg[0] * T.r1[0] + sg[1] * T.r1[1] + sg[2] * T.r1[2];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        double lap = T.b3[nf_i3(0, 0, a)] + T.b3[nf_i3(1, 1, a)] + T.b3[nf_i3(2, 2, a)];
+        double q = sg[0] * T.b3[nf_i3(a, 0, 2)] + sg[1] * T.b3[nf_i3(a, 1, 2)] + sg[2] * T.b3[nf_i3(a, 2, 2)];
+        disp[a] += rlam * sg[2] * (2.0 * T.r1[a] - c1 * lap) +
+                   rmu * (-2.0 * c1 * q + 2.0 * sg[a] * T.r1[2] + (a == 2 ? 2.0 * sr : 0.0));
     }
-    // back to global coordinates: disp_g = U disp, H_g = U H U^T; scale 1/(2mu)
-    double s = 0.5 / rmu;
+}
+
+// Double layer, strain. Reference assembly (src/elfmm3dtria.f:2636-2660,
+// 2692-2714): with d(a,k,b,m) = -c1 b(a,b,k,m) + 2 d_ak r(b,m) and
+// S(a,k,b) = lam d_k3 sum_l d(l,a,l,b) + mu (d(k,a,3,b) + d(3,a,k,b)),
+// strain_ab = -sym_ab sum_k s_k S(a,k,b). Summing over k first:
+// lam s_z (2 r(a,b) - c1 sum_l b(l,l,a,b))
+// + mu (-2 c1 sum_k s_k b(a,b,k,z) + s_a r(z,b) + s_b r(z,a)
+//       + d_az (s.r)_b + d_bz (s.r)_a)
+ELFMM_HD void nf_dl_high(const TablesHigh &T, const double *sg, double c1, double rlam, double rmu,
+                         double *H)
+{
+    double sr[3];
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+        sr[b] = sg[0] * T.r2[nf_i2(0, b)] + sg[1] * T.r2[nf_i2(1, b)] + sg[2] * T.r2[nf_i2(2, b)];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = a; b < 3; b++) {
+            double lap = T.b4[nf_i4(0, 0, a, b)] + T.b4[nf_i4(1, 1, a, b)] + T.b4[nf_i4(2, 2, a, b)];
+            double q = sg[0] * T.b4[nf_i4(a, b, 0, 2)] + sg[1] * T.b4[nf_i4(a, b, 1, 2)] +
+                       sg[2] * T.b4[nf_i4(a, b, 2, 2)];
+            double v = rlam * sg[2] * (2.0 * T.r2[nf_i2(a, b)] - c1 * lap) +
+                       rmu * (-2.0 * c1 * q + sg[a] * T.r2[nf_i2(2, b)] + sg[b] * T.r2[nf_i2(2, a)] +
+                              (a == 2 ? sr[b] : 0.0) + (b == 2 ? sr[a] : 0.0));
+            H[nf_i2(a, b)] -= v;
+        }
+}
+
+// Local displacement and strain back to global coordinates, scaled by
+// s: out[0..2] += s U disp, out[3..8] += s (U H U^T) as xx yy zz xy xz yz.
+ELFMM_HD void nf_to_global(const double *U, const double *disp, const double *H, double s, double *out)
+{
     double g[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) g[i] = U[i] * disp[0] + U[3 + i] * disp[1] + U[6 + i] * disp[2];
@@ -441,6 +430,84 @@ ELFMM_HD void nf_pair(const double *rec, const double *P, int iself, double rlam
     out[2] += s * g[2];
 #pragma unroll
     for (int q = 0; q < 6; q++) out[3 + q] += s * G[q];
+}
+
+// Point coordinates in the triangle's local frame.
+ELFMM_HD void nf_local_point(const double *rec, const double *P, double &x0, double &y0, double &z0)
+{
+    const double *U = rec;
+    double q0 = P[0] - rec[9], q1 = P[1] - rec[10], q2 = P[2] - rec[11];
+    x0 = U[0] * q0 + U[1] * q1 + U[2] * q2;
+    y0 = U[3] * q0 + U[4] * q1 + U[5] * q2;
+    z0 = U[6] * q0 + U[7] * q1 + U[8] * q2;
+}
+
+// One (target, triangle) pair: accumulate displacement (3) and strain
+// (xx yy zz xy xz yz) in global coordinates into out[9]; c1 =
+// (lam+mu)/(lam+2mu); ifsl/ifdl select the layers. Two passes over the
+// edges keep the live state small: orders <= 3 (single layer, double-layer
+// displacement), then order 4 (double-layer strain).
+ELFMM_HD void nf_pair(const double *rec, const double *P, int iself, double rlam, double rmu,
+                      double c1, int ifsl, int ifdl, double *out)
+{
+    double x0, y0, z0;
+    nf_local_point(rec, P, x0, y0, z0);
+    double disp[3] = {0, 0, 0};
+    double H[6] = {0, 0, 0, 0, 0, 0};   // local xx xy xz yy yz zz
+    {
+        TablesLow T;
+        nf_tables_low(rec[12], rec[13], rec[14], x0, y0, z0, iself, T);
+        if (ifsl) nf_sl_low(T, rec + 15, c1, disp, H);
+        if (ifdl) nf_dl_low(T, rec + 18, c1, rlam, rmu, disp);
+    }
+    if (ifdl) {
+        TablesHigh T;
+        nf_tables_high(rec[12], rec[13], rec[14], x0, y0, z0, iself, T);
+        nf_dl_high(T, rec + 18, c1, rlam, rmu, H);
+    }
+    nf_to_global(rec, disp, H, 0.5 / rmu, out);
+}
+
+// The same pair for the unit densities e_k (global frame), k = 0..2:
+// emit(layer, k, v) receives the 9 fields (displacement, strain xx yy zz
+// xy xz yz) of layer 0 (single) or 1 (double) with density e_k. The tables
+// are computed once; the fields are linear in the density, so
+// sum_k sigma_k v[layer][k] is the pair's contribution.
+template <class Emit>
+ELFMM_HD void nf_pair_basis(const double *rec, const double *P, int iself, double rlam, double rmu,
+                            double c1, int ifsl, int ifdl, Emit &&emit)
+{
+    double x0, y0, z0;
+    nf_local_point(rec, P, x0, y0, z0);
+    double dld[3][3];                   // double-layer local displacement per k
+    {
+        TablesLow T;
+        nf_tables_low(rec[12], rec[13], rec[14], x0, y0, z0, iself, T);
+        for (int k = 0; k < 3; k++) {
+            double sg[3] = {rec[k], rec[3 + k], rec[6 + k]};   // e_k in the local frame
+            if (ifsl) {
+                double disp[3] = {0, 0, 0}, H[6] = {0, 0, 0, 0, 0, 0}, v[9];
+                nf_sl_low(T, sg, c1, disp, H);
+                for (int q = 0; q < 9; q++) v[q] = 0.0;
+                nf_to_global(rec, disp, H, 0.5 / rmu, v);
+                emit(0, k, v);
+            }
+            dld[k][0] = dld[k][1] = dld[k][2] = 0.0;
+            if (ifdl) nf_dl_low(T, sg, c1, rlam, rmu, dld[k]);
+        }
+    }
+    if (ifdl) {
+        TablesHigh T;
+        nf_tables_high(rec[12], rec[13], rec[14], x0, y0, z0, iself, T);
+        for (int k = 0; k < 3; k++) {
+            double sg[3] = {rec[k], rec[3 + k], rec[6 + k]};
+            double H[6] = {0, 0, 0, 0, 0, 0}, v[9];
+            nf_dl_high(T, sg, c1, rlam, rmu, H);
+            for (int q = 0; q < 9; q++) v[q] = 0.0;
+            nf_to_global(rec, dld[k], H, 0.5 / rmu, v);
+            emit(1, k, v);
+        }
+    }
 }
 
 }  // namespace elfmm

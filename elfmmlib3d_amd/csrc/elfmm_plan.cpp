@@ -1,0 +1,336 @@
+// Host planner, part 2: the task lists of one apply, built from the octree
+// on the host worker pool.
+//
+// Which boxes take part in each step follows the reference's
+// lfmm3dtrilhesstargmain (src/lfmm3dtrilhess.f:517-1215) and lfmm3d_list2
+// (src/lfmm3drouts.f:297-496): multipoles of sourced leaves at level >= 2,
+// M2M/L2L between sourced boxes, M2L over list 2 from level 2 down (target
+// boxes without targets pruned when no source outputs are requested), P2L
+// over list 4, M2T over list 3, L2T at leaves of level >= 2, near field over
+// self + list 1. Each translation step is grouped by operator so that one
+// dense matrix serves many (source, destination) pairs.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+
+#include "elfmm_internal.h"
+
+namespace elfmm {
+namespace {
+
+void add_pairs(GemmLaunch &g, int64_t a_off, int dim, const int *src, const int *dst, const double *scale,
+               int np)
+{
+    int base = (int)g.src.size();
+    g.src.insert(g.src.end(), src, src + np);
+    g.dst.insert(g.dst.end(), dst, dst + np);
+    g.scale.insert(g.scale.end(), scale, scale + np);
+    for (int p0 = 0; p0 < np; p0 += GT_P)
+        for (int r0 = 0; r0 < dim; r0 += GT_M)
+            g.tiles.push_back({a_off, dim, r0, base + p0, std::min(GT_P, np - p0)});
+}
+
+// octant of child c in parent b: bit d set when c is the upper half in d
+inline int octant(const Tree &T, int c, int b)
+{
+    int o = 0;
+    for (int d = 0; d < 3; d++)
+        if (T.ijk[3 * c + d] != 2 * (T.ijk[3 * b + d] - 1) + 1) o |= 1 << d;
+    return o;
+}
+
+inline int offset_index(const Tree &T, int j, int b)
+{
+    return (T.ijk[3 * j] - T.ijk[3 * b] + 3) * 49 + (T.ijk[3 * j + 1] - T.ijk[3 * b + 1] + 3) * 7 +
+           (T.ijk[3 * j + 2] - T.ijk[3 * b + 2] + 3);
+}
+
+// M2M (up) or L2L (down) launch for one level: pairs grouped by octant.
+GemmLaunch tree_pass(const Tree &T, int lev, int nc, bool up)
+{
+    GemmLaunch g;
+    std::vector<int> s[8], d[8];
+    for (int b = T.laddr[2 * lev]; b < T.laddr[2 * lev] + T.laddr[2 * lev + 1]; b++) {
+        if (!T.nkids[b] || (up && T.src_count[b] == 0)) continue;
+        for (int q = 0; q < T.nkids[b]; q++) {
+            int c = T.kids[8 * b + q];
+            if (up && T.src_count[c] == 0) continue;
+            int o = octant(T, c, b);
+            s[o].push_back(up ? c : b);
+            d[o].push_back(up ? b : c);
+        }
+    }
+    for (int o = 0; o < 8; o++)
+        if (!s[o].empty()) {
+            std::vector<double> one(s[o].size(), 1.0);
+            add_pairs(g, (int64_t)o * nc * nc, nc, s[o].data(), d[o].data(), one.data(), (int)s[o].size());
+        }
+    return g;
+}
+
+constexpr int CHUNK = 64;
+
+}  // namespace
+
+void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool direct_only, int nrhs,
+                Plan &P, bool iter)
+{
+    int nb = T.nboxes;
+    int p = ops ? ops->p : 0;
+    int nc = ncoef(p);
+    int nchunk = (nb + CHUNK - 1) / CHUNK;
+    P = Plan();
+    for (double &v : P.stats) v = 0;
+    P.sides.resize(nb);
+    for (int b = 0; b < nb; b++) P.sides[b] = T.size / (double)(1 << T.level[b]);
+    auto npts = [&](int b) { return (src_eval ? T.src_count[b] : 0) + T.tgt_count[b]; };
+
+    if (ops) {
+        for (int b = 0; b < nb; b++)
+            if (!T.nkids[b] && T.level[b] >= 2 && T.src_count[b] > 0) P.p2m_boxes.push_back(b);
+        // M2M children at lev+1 -> parents at lev, for lev = nlev-1 .. 2;
+        // L2L parents at lev -> children, for lev = 2 .. nlev-1
+        for (int lev = T.nlev - 1; lev >= 2; lev--) {
+            GemmLaunch g = tree_pass(T, lev, nc, true);
+            if (!g.tiles.empty()) P.m2m_l.push_back(std::move(g));
+        }
+        for (int lev = 2; lev <= T.nlev - 1; lev++) {
+            GemmLaunch g = tree_pass(T, lev, nc, false);
+            if (!g.tiles.empty()) P.l2l_l.push_back(std::move(g));
+        }
+
+        // M2L over list 2, grouped by offset: per-chunk counts by offset,
+        // then every chunk fills its slice of each offset's run, so the
+        // pairs of one offset stay in increasing target-box order.
+        std::vector<int> cnt(343 * (size_t)nchunk, 0);
+        auto m2l_target = [&](int b) { return T.level[b] >= 2 && (src_eval || T.tgt_count[b] > 0); };
+        parallel_chunks(nchunk, [&](int ch, int) {
+            int *c = &cnt[343 * (size_t)ch];
+            for (int b = ch * CHUNK; b < std::min(nb, (ch + 1) * CHUNK); b++) {
+                if (!m2l_target(b)) continue;
+                for (int q = T.lstart[2][b]; q < T.lstart[2][b + 1]; q++) {
+                    int j = T.litems[2][q];
+                    if (T.src_count[j]) c[offset_index(T, j, b)]++;
+                }
+            }
+        });
+        std::vector<int> ostart(344, 0);
+        for (int o = 0; o < 343; o++) {
+            int run = ostart[o];
+            for (int ch = 0; ch < nchunk; ch++) {
+                int v = cnt[343 * (size_t)ch + o];
+                cnt[343 * (size_t)ch + o] = run;
+                run += v;
+            }
+            ostart[o + 1] = run;
+        }
+        int npair = ostart[343];
+        std::vector<int> ps(npair), pd(npair);
+        std::vector<double> psc(npair);
+        parallel_chunks(nchunk, [&](int ch, int) {
+            int *pos = &cnt[343 * (size_t)ch];
+            for (int b = ch * CHUNK; b < std::min(nb, (ch + 1) * CHUNK); b++) {
+                if (!m2l_target(b)) continue;
+                double isb = 1.0 / P.sides[b];
+                for (int q = T.lstart[2][b]; q < T.lstart[2][b + 1]; q++) {
+                    int j = T.litems[2][q];
+                    if (!T.src_count[j]) continue;
+                    int k = pos[offset_index(T, j, b)]++;
+                    ps[k] = j;
+                    pd[k] = b;
+                    psc[k] = isb;
+                }
+            }
+        });
+        // work counts: dense operator flops as executed and the reference's
+        // point-and-shoot count per pair and right-hand side, two rotations
+        // R(p) = 4 sum (2n+1)^2 and a z-shift Z(p) = 6 sum_j sum_k (p+1-|k|)
+        std::vector<double> alg_m2l(p + 1, 0.0);
+        for (int pt = 0; pt <= p; pt++) {
+            double R = 0, Z = 0;
+            for (int a = 0; a <= pt; a++) R += 4.0 * (2 * a + 1) * (2 * a + 1);
+            for (int jj = 0; jj <= pt; jj++)
+                for (int kk = -jj; kk <= jj; kk++) Z += 6.0 * (pt + 1 - std::abs(kk));
+            alg_m2l[pt] = 2 * R + Z;
+        }
+        double dense = 0, alg = 0;
+        // the pairs are already in offset order; tiles per offset cover
+        // GT_P pairs by GT_M rows of that offset's operator
+        std::vector<int64_t> tstart(344, 0);
+        for (int o = 0; o < 343; o++) {
+            int m = ostart[o + 1] - ostart[o];
+            int dim = ncoef(ops->m2l_pt[o]);
+            tstart[o + 1] = tstart[o] + (m ? (int64_t)((m + GT_P - 1) / GT_P) * ((dim + GT_M - 1) / GT_M) : 0);
+            if (!m) continue;
+            dense += m * 2.0 * dim * (double)dim * nrhs;
+            alg += m * alg_m2l[ops->m2l_pt[o]] * nrhs;
+        }
+        P.m2l.tiles.resize(tstart[343]);
+        parallel_chunks(343, [&](int o, int) {
+            int m = ostart[o + 1] - ostart[o];
+            if (!m) return;
+            int dim = ncoef(ops->m2l_pt[o]);
+            GemmTile *t = &P.m2l.tiles[tstart[o]];
+            for (int p0 = 0; p0 < m; p0 += GT_P)
+                for (int r0 = 0; r0 < dim; r0 += GT_M)
+                    *t++ = GemmTile{ops->m2l_off[o], dim, r0, ostart[o] + p0, std::min(GT_P, m - p0)};
+        });
+        P.m2l.src.swap(ps);
+        P.m2l.dst.swap(pd);
+        P.m2l.scale.swap(psc);
+        P.stats[3] = npair;
+        P.stats[4] = dense;
+        P.stats[5] = alg;
+
+        // P2L: local of box j from the sources of i with j in list3(i)
+        // (src/lfmm3dtrilhess.f:760-829); owner j, sources list4(j).
+        // M2T: multipole of i at the points of j with j in list4(i)
+        // (src/lfmm3dtrilhess.f:864-981); owner j, sources list3(j).
+        P.p2l_start.push_back(0);
+        P.m2t_start.push_back(0);
+        for (int j = 0; j < nb && !iter; j++) {
+            int c = 0;
+            for (int q = T.lstart[4][j]; q < T.lstart[4][j + 1]; q++) {
+                int i = T.litems[4][q];
+                if (!T.src_count[i]) continue;
+                P.p2l_items.push_back(i);
+                c++;
+            }
+            if (c) {
+                P.p2l_boxes.push_back(j);
+                P.p2l_start.push_back((int)P.p2l_items.size());
+            }
+            if (!npts(j)) continue;
+            c = 0;
+            for (int q = T.lstart[3][j]; q < T.lstart[3][j + 1]; q++) {
+                int i = T.litems[3][q];
+                if (!T.src_count[i]) continue;
+                P.m2t_items.push_back(i);
+                c++;
+            }
+            if (c) {
+                P.m2t_boxes.push_back(j);
+                P.m2t_start.push_back((int)P.m2t_items.size());
+            }
+        }
+    }
+
+    // leaves: L2T (level >= 2) and near field over self + list 1. With
+    // iter, lists 3 and 4 join the near field; the reference evaluates a
+    // box's list 4 at every point below it, so a leaf also takes the list 4
+    // of each of its ancestors and every point is handled once, by its leaf.
+    P.nf_start.push_back(0);
+    for (int b = 0; b < nb; b++) {
+        if (T.nkids[b] || !npts(b)) continue;
+        if (ops && T.level[b] >= 2) P.l2t_boxes.push_back(b);
+        size_t before = P.nf_items.size();
+        for (int q = T.lstart[1][b]; q < T.lstart[1][b + 1]; q++) {
+            int j = T.litems[1][q];
+            if (T.src_count[j]) P.nf_items.push_back(j);
+        }
+        if (T.src_count[b]) P.nf_items.push_back(b);
+        if (iter) {
+            for (int q = T.lstart[3][b]; q < T.lstart[3][b + 1]; q++) {
+                int j = T.litems[3][q];
+                if (T.src_count[j]) P.nf_items.push_back(j);
+            }
+            for (int a = b; a >= 0; a = T.parent[a])
+                for (int q = T.lstart[4][a]; q < T.lstart[4][a + 1]; q++) {
+                    int j = T.litems[4][q];
+                    if (T.src_count[j]) P.nf_items.push_back(j);
+                }
+        }
+        if (P.nf_items.size() == before) continue;
+        P.nf_boxes.push_back(b);
+        P.nf_start.push_back((int)P.nf_items.size());
+    }
+    if (direct_only) P.nf_boxes.clear();
+
+    // evaluation tasks: L2T one per point of each leaf; M2T one per (point,
+    // list-3 box), grouped by the box whose derivative fields are read
+    auto fill_points = [&](int b, int slot, int *tp, int *ts) {
+        int k = 0;
+        if (src_eval)
+            for (int i = T.src_start[b]; i < T.src_start[b] + T.src_count[b]; i++, k++) {
+                tp[k] = i;
+                ts[k] = slot;
+            }
+        for (int i = T.tgt_start[b]; i < T.tgt_start[b] + T.tgt_count[b]; i++, k++) {
+            tp[k] = n + i;
+            ts[k] = slot;
+        }
+    };
+    {
+        int nl = (int)P.l2t_boxes.size();
+        std::vector<int> off(nl + 1, 0);
+        for (int q = 0; q < nl; q++) off[q + 1] = off[q] + npts(P.l2t_boxes[q]);
+        P.l2t_tpt.resize(off[nl]);
+        P.l2t_tslot.resize(off[nl]);
+        parallel_chunks((nl + CHUNK - 1) / CHUNK, [&](int ch, int) {
+            for (int q = ch * CHUNK; q < std::min(nl, (ch + 1) * CHUNK); q++)
+                fill_points(P.l2t_boxes[q], q, &P.l2t_tpt[off[q]], &P.l2t_tslot[off[q]]);
+        });
+    }
+    if (!P.m2t_items.empty()) {
+        // slots in order of first use; the owners of each slot in order
+        std::vector<int> slot_of(nb, -1), nown;
+        for (size_t q = 0; q < P.m2t_boxes.size(); q++)
+            for (int k = P.m2t_start[q]; k < P.m2t_start[q + 1]; k++) {
+                int i = P.m2t_items[k];
+                if (slot_of[i] < 0) {
+                    slot_of[i] = (int)P.m2t_slot_box.size();
+                    P.m2t_slot_box.push_back(i);
+                    nown.push_back(0);
+                }
+                nown[slot_of[i]]++;
+            }
+        int ns = (int)P.m2t_slot_box.size();
+        std::vector<int> ostart(ns + 1, 0);
+        for (int sl = 0; sl < ns; sl++) ostart[sl + 1] = ostart[sl] + nown[sl];
+        std::vector<int> owners(ostart[ns]), fill(ostart.begin(), ostart.end() - 1);
+        for (size_t q = 0; q < P.m2t_boxes.size(); q++)
+            for (int k = P.m2t_start[q]; k < P.m2t_start[q + 1]; k++)
+                owners[fill[slot_of[P.m2t_items[k]]]++] = P.m2t_boxes[q];
+        std::vector<int64_t> toff(ns + 1, 0);
+        for (int sl = 0; sl < ns; sl++) {
+            int64_t t = 0;
+            for (int k = ostart[sl]; k < ostart[sl + 1]; k++) t += npts(owners[k]);
+            toff[sl + 1] = toff[sl] + t;
+        }
+        P.m2t_tpt.resize(toff[ns]);
+        P.m2t_tslot.resize(toff[ns]);
+        parallel_chunks((ns + CHUNK - 1) / CHUNK, [&](int ch, int) {
+            for (int sl = ch * CHUNK; sl < std::min(ns, (ch + 1) * CHUNK); sl++) {
+                int64_t at = toff[sl];
+                for (int k = ostart[sl]; k < ostart[sl + 1]; k++) {
+                    fill_points(owners[k], sl, &P.m2t_tpt[at], &P.m2t_tslot[at]);
+                    at += npts(owners[k]);
+                }
+            }
+        });
+    }
+
+    // remaining work counts (elfmm_gfx950_last_stats)
+    P.stats[0] = nb;
+    P.stats[1] = T.nlev;
+    P.stats[2] = p;
+    double npair_near = 0;
+    for (size_t q = 0; q < P.nf_boxes.size(); q++) {
+        double ns = 0;
+        for (int k = P.nf_start[q]; k < P.nf_start[q + 1]; k++) ns += T.src_count[P.nf_items[k]];
+        npair_near += npts(P.nf_boxes[q]) * ns;
+    }
+    P.stats[6] = npair_near;
+    P.stats[7] = (double)P.m2l.tiles.size();
+    double m2m_pairs = 0, l2l_pairs = 0;
+    for (auto &g : P.m2m_l) m2m_pairs += g.src.size();
+    for (auto &g : P.l2l_l) l2l_pairs += g.src.size();
+    P.stats[8] = m2m_pairs;
+    P.stats[9] = l2l_pairs;
+    P.stats[10] = (double)P.p2m_boxes.size();
+    P.stats[11] = (double)P.p2l_items.size();
+    P.stats[12] = (double)P.m2t_items.size();
+    P.stats[13] = (double)P.l2t_boxes.size();
+}
+
+}  // namespace elfmm

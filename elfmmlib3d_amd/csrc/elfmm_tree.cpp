@@ -48,6 +48,22 @@ int iprec_nbox(int iprec, int npts)
     }
 }
 
+int iprec_nbox_iter(int iprec, int npts)
+{
+    // src/lfmm3dtrilhess_iter_1.f:246-255 (bscale = 2)
+    switch (iprec) {
+    case -2: return 16;
+    case -1: return 30;
+    case 0: return 60;
+    case 1: return 120;
+    case 2: return 240;
+    case 3: return 480;
+    case 4: return 960;
+    case 5: return 1400;
+    default: return npts;
+    }
+}
+
 // Decay of (sqrt(3)/2)^n / z1^(n+1) below eps (src/l3dterms.f:42-85).
 static int terms_for(double eps, double z1)
 {

@@ -1,10 +1,10 @@
 ! Fortran interfaces of libelfmm_gfx950.so.
 !
-! The drop-in entry points elfmm3dtria, elfmm3dtriatarg and el3dtriadirecttarg
-! are exported under the reference's own linker names (lowercase, trailing
+! The drop-in entry points elfmm3dtria, elfmm3dtriatarg, el3dtriadirecttarg,
+! elfmm3dtriaiter and elfmm3dtriatargiter are exported under the reference's own linker names (lowercase, trailing
 ! underscore, every argument by reference), so a program that called the
 ! reference's subroutines through implicit interfaces links this library
-! unchanged. This module adds explicit interfaces for those three (argument
+! unchanged. This module adds explicit interfaces for those (argument
 ! checking at compile time) and bind(C) interfaces for the device-resident
 ! entry point and the diagnostics.
 module elfmm_gfx950
@@ -55,6 +55,46 @@ module elfmm_gfx950
        real(c_double) :: ptfrc(3, *), strain(3, 3, *)
        real(c_double) :: target(3, *), ptfrctarg(3, *), straintarg(3, 3, *)
      end subroutine el3dtriadirecttarg
+
+     ! src/elfmm3dtria_iter_1.f:160-166 (wsave carries a handle to the
+     ! device-resident near-field interactions; lused = 4 for icomp 1-5)
+     subroutine elfmm3dtriatargiter(ier, iprec, rlam, rmu, triangle, trinorm, nparts, source, &
+          ifsingle, sigma_sl, ifdouble, sigma_dl, ifptfrc, ptfrc, ifstrain, strain, &
+          ntargs, target, ifptfrctarg, ptfrctarg, ifstraintarg, straintarg, &
+          icomp_type, wsave, lwsave, lused) bind(C, name="elfmm3dtriatargiter_")
+       import :: c_int, c_double
+       integer(c_int) :: ier, lused
+       integer(c_int) :: iprec, nparts, ifsingle, ifdouble, ifptfrc, ifstrain
+       integer(c_int) :: ntargs, ifptfrctarg, ifstraintarg, icomp_type, lwsave
+       real(c_double) :: rlam, rmu
+       real(c_double) :: triangle(3, 3, *), trinorm(3, *), source(3, *)
+       real(c_double) :: sigma_sl(3, *), sigma_dl(3, *)
+       real(c_double) :: ptfrc(3, *), strain(3, 3, *)
+       real(c_double) :: target(3, *), ptfrctarg(3, *), straintarg(3, 3, *)
+       real(c_double) :: wsave(*)
+     end subroutine elfmm3dtriatargiter
+
+     ! src/elfmm3dtria_iter_1.f:43-47
+     subroutine elfmm3dtriaiter(ier, iprec, rlam, rmu, triangle, trinorm, nparts, source, &
+          ifsingle, sigma_sl, ifdouble, sigma_dl, ifptfrc, ptfrc, ifstrain, strain, &
+          icomp_type, wsave, lwsave, lused) bind(C, name="elfmm3dtriaiter_")
+       import :: c_int, c_double
+       integer(c_int) :: ier, lused
+       integer(c_int) :: iprec, nparts, ifsingle, ifdouble, ifptfrc, ifstrain, icomp_type, lwsave
+       real(c_double) :: rlam, rmu
+       real(c_double) :: triangle(3, 3, *), trinorm(3, *), source(3, *)
+       real(c_double) :: sigma_sl(3, *), sigma_dl(3, *)
+       real(c_double) :: ptfrc(3, *), strain(3, 3, *)
+       real(c_double) :: wsave(*)
+     end subroutine elfmm3dtriaiter
+
+     ! Frees the device-resident interactions named by wsave.
+     integer(c_int) function elfmm_gfx950_iter_release(wsave, lwsave) &
+          bind(C, name="elfmm_gfx950_iter_release")
+       import :: c_int, c_double
+       real(c_double) :: wsave(*)
+       integer(c_int), value :: lwsave
+     end function elfmm_gfx950_iter_release
 
      ! Device-resident elfmm3dtriatarg: every array argument is a device
      ! address (type(c_ptr)), stream is a hipStream_t (c_null_ptr = default).

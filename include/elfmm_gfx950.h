@@ -56,6 +56,52 @@ void el3dtriadirecttarg_(const double *rlam, const double *rmu, const double *tr
                          const double *target, const int *ifptfrctarg, double *ptfrctarg,
                          const int *ifstraintarg, double *straintarg);
 
+/* Replaces elfmm3dtriatargiter (src/elfmm3dtria_iter_1.f:160-166), the iter
+ * engine (leaf size twice elfmm3dtriatarg's, lists 3 and 4 in the near
+ * field, half-space root rule of the link set the reference builds it in).
+ * icomp_type: 1 compute; 2 compute and keep the near-field interactions; 3
+ * compute the far field and replay the kept interactions with the current
+ * densities (same geometry and flags as the icomp 2 call); 4 plan only, set
+ * lused; 5 keep the interactions and add the near field to the outputs
+ * (no far field); 0 nothing (lused = the reference's work-array size).
+ * The interactions are kept in device memory; wsave (lwsave >= lused words,
+ * lused = 4 for icomp 1-5) carries their handle between calls. ier = 64:
+ * icomp 3 found no interactions matching wsave and computed the near field
+ * directly. lwsave below lused with icomp 2/3/5 also computes directly. */
+void elfmm3dtriatargiter_(int *ier, const int *iprec, const double *rlam, const double *rmu,
+                          const double *triangle, const double *trinorm, const int *nparts,
+                          const double *source, const int *ifsingle, const double *sigma_sl,
+                          const int *ifdouble, const double *sigma_dl, const int *ifptfrc,
+                          double *ptfrc, const int *ifstrain, double *strain, const int *ntargs,
+                          const double *target, const int *ifptfrctarg, double *ptfrctarg,
+                          const int *ifstraintarg, double *straintarg, const int *icomp_type,
+                          double *wsave, const int *lwsave, int *lused);
+
+/* Replaces elfmm3dtriaiter (src/elfmm3dtria_iter_1.f:43-47): as above at the
+ * centroids only. */
+void elfmm3dtriaiter_(int *ier, const int *iprec, const double *rlam, const double *rmu,
+                      const double *triangle, const double *trinorm, const int *nparts,
+                      const double *source, const int *ifsingle, const double *sigma_sl,
+                      const int *ifdouble, const double *sigma_dl, const int *ifptfrc,
+                      double *ptfrc, const int *ifstrain, double *strain, const int *icomp_type,
+                      double *wsave, const int *lwsave, int *lused);
+
+/* Device-resident elfmm3dtriatargiter (device arrays, host wsave), same
+ * semantics; returns ier. No counterpart in the reference. */
+int elfmm_gfx950_triatargiter_dev(int iprec, double rlam, double rmu, int nparts,
+                                  const double *d_triangle, const double *d_trinorm,
+                                  const double *d_source, int ifsingle, const double *d_sigma_sl,
+                                  int ifdouble, const double *d_sigma_dl, int ifptfrc,
+                                  double *d_ptfrc, int ifstrain, double *d_strain, int ntargs,
+                                  const double *d_target, int ifptfrctarg, double *d_ptfrctarg,
+                                  int ifstraintarg, double *d_straintarg, int icomp_type,
+                                  double *wsave, int lwsave, int *lused, void *stream);
+
+/* Frees the device-resident interactions named by wsave (returns 1 if
+ * freed). No counterpart in the reference, whose wsave is the caller's
+ * memory. */
+int elfmm_gfx950_iter_release(const double *wsave, int lwsave);
+
 /* Device-resident elfmm3dtriatarg: all arrays are device pointers with the
  * Fortran layouts above; the call is enqueued on `stream` (a hipStream_t,
  * NULL for the default stream) and returns after the outputs are written.
@@ -84,7 +130,8 @@ void elfmm_gfx950_last_timings(double *ms16);
  * rotate/shift/rotate scheme for the same pairs, [6] near-field (point,
  * triangle) pairs, [7] M2L tiles, [8] M2M pairs, [9] L2L pairs, [10] P2M
  * boxes, [11] P2L (box, source box) pairs, [12] M2T (box, source box) pairs,
- * [13] L2T boxes, [14]-[15] reserved. */
+ * [13] L2T boxes, [14] iter: near-field store size in the reference's
+ * word layout (its lused), [15] iter: this library's store size in words. */
 void elfmm_gfx950_last_stats(double *st16);
 
 /* Library version string. */

@@ -220,3 +220,61 @@ def tree(source, target, nbox):
     return dict(boxes=boxes, centers=centers, iz=iz.copy(), iztarg=izt[:m].copy(),
                 laddr=laddr.copy(), nlev=nlev.value, center=center.copy(),
                 size=size.value, lists=lists)
+
+
+# ---------------------------------------------------------------------------
+# Half-space link set (oracle/_ref/libelfmm_refh.so, `make -C oracle refh`):
+# the iter entry points, built as the reference builds them, with the
+# half-space root rule of d3tstrcr_mindlin.f.
+
+REFH_SO = os.path.join(HERE, "_ref", "libelfmm_refh.so")
+_libh = None
+
+
+def available_h():
+    return os.path.exists(REFH_SO)
+
+
+def libh():
+    global _libh
+    if _libh is None:
+        if not available_h():
+            raise RuntimeError(f"reference library not built: {REFH_SO}")
+        _libh = C.CDLL(REFH_SO)
+    return _libh
+
+
+def elfmm3dtriatargiter(iprec, rlam, rmu, triangle, trinorm, source,
+                        sigma_sl=None, sigma_dl=None, target=None,
+                        ifptfrc=1, ifstrain=1, ifptfrctarg=1, ifstraintarg=1,
+                        icomp_type=1, wsave=None):
+    """Reference iter FMM (src/elfmm3dtria_iter_1.f:160). Returns
+    (ier, lused, ptfrc, strain, ptfrctarg, straintarg, wsave); wsave is the
+    caller's array (allocated with the returned lused for icomp 2/3)."""
+    n = triangle.shape[0]
+    tri = _f64(triangle, (n, 3, 3))
+    nrm = _f64(trinorm, (n, 3))
+    src = _f64(source, (n, 3))
+    ifs = 0 if sigma_sl is None else 1
+    ifd = 0 if sigma_dl is None else 1
+    ssl = _f64(sigma_sl if ifs else np.zeros((n, 3)), (n, 3))
+    sdl = _f64(sigma_dl if ifd else np.zeros((n, 3)), (n, 3))
+    if target is None:
+        target = np.zeros((0, 3))
+    m = target.shape[0]
+    tgt = _f64(target if m else np.zeros((1, 3)), (max(m, 1), 3))
+    pt = np.zeros((n, 3))
+    st = np.zeros((n, 3, 3))
+    ptt = np.zeros((max(m, 1), 3))
+    stt = np.zeros((max(m, 1), 3, 3))
+    if wsave is None:
+        wsave = np.zeros(1)
+    ier = C.c_int(0)
+    lused = C.c_int(0)
+    libh().elfmm3dtriatargiter_(
+        C.byref(ier), _i(iprec), _d(rlam), _d(rmu), _p(tri), _p(nrm), _i(n),
+        _p(src), _i(ifs), _p(ssl), _i(ifd), _p(sdl), _i(ifptfrc), _p(pt),
+        _i(ifstrain), _p(st), _i(m), _p(tgt), _i(ifptfrctarg if m else 0), _p(ptt),
+        _i(ifstraintarg if m else 0), _p(stt), _i(icomp_type), _p(wsave), _i(wsave.size),
+        C.byref(lused))
+    return ier.value, lused.value, pt, st, ptt[:m], stt[:m], wsave

@@ -46,3 +46,24 @@ extern "C" void tree_get(int *boxes, double *centers, int *iz, int *iztarg, int 
         litems += T.litems[t].size();
     }
 }
+
+// Near-field store size of the iter engine in the reference's word layout
+// (its lused for icomp 4): plan the iter near lists on the tree just built.
+extern "C" double plan_iter_ref_words(int n, int ifsl, int ifdl, int ifpt, int ifst, int ifptt, int ifstt)
+{
+    const elfmm::Tree &T = g_tree;
+    elfmm::Plan P;
+    bool src_eval = ifpt || ifst;
+    elfmm::build_plan(T, n, nullptr, src_eval, false, 4, P, true);
+    double words = 0;
+    int nl = ifsl + ifdl;
+    for (size_t q = 0; q < P.nf_boxes.size(); q++) {
+        int b = P.nf_boxes[q];
+        double ntri = 0;
+        for (int k = P.nf_start[q]; k < P.nf_start[q + 1]; k++) ntri += T.src_count[P.nf_items[k]];
+        words += nl * ntri *
+                 ((src_eval ? T.src_count[b] : 0) * 3.0 * (3 * ifpt + 9 * ifst) +
+                  T.tgt_count[b] * 3.0 * (3 * ifptt + 9 * ifstt));
+    }
+    return words;
+}

@@ -17,6 +17,11 @@ never need the reference itself. Cases (all FP64):
   fmm_s6_m1     fmm_s6 geometry at iprec -1
   fmm_fault     4 planar faults (~3000 right triangles, open surfaces) with
                 1000 targets around them, lam=1, mu=0.8, iprec 2
+  iter_c1_p2    elfmm3dtriatargiter (half-space link set, d3tstrcr_mindlin
+                root) on the C1 sphere moved to centre (0,0,-2) with 1000
+                shell targets, lam=1.2 mu=0.7, iprec 2: icomp 4 (lused),
+                icomp 1 outputs for two density sets (sigma, sigma2)
+  iter_c1_p1    the same at iprec 1
   pairs         500 random (target, triangle) pairs incl. self and
                 near-coplanar targets: elust3/eltst3triadirecttarg_one
   tables        random plane-triangle tables (triabtable / triartable)
@@ -75,6 +80,26 @@ def fault_case(seed=13, ntri=3000, ntarg=1000):
     return tri, nrm, src, ssl, sdl, tg
 
 
+def iter_case(name, iprec, lam=1.2, mu=0.7):
+    tri, nrm, src, ssl, sdl, tg = sphere_case(12, 1000, 23)
+    shift = np.array([0.0, 0.0, -2.0])
+    tri, src, tg = tri + shift, src + shift, tg + shift
+    rng = np.random.default_rng(29)
+    ssl2, sdl2 = rng.uniform(-1, 1, ssl.shape), rng.uniform(-1, 1, sdl.shape)
+    ier, lused, *_ = ref.elfmm3dtriatargiter(iprec, lam, mu, tri, nrm, src, ssl, sdl, tg, icomp_type=4)
+    assert ier == 0
+    # the reference computes its near field only when lwsave >= lused
+    w = np.zeros(lused)
+    out = {}
+    for tag, (a, b) in (("", (ssl, sdl)), ("2", (ssl2, sdl2))):
+        ier, _, pt, st, ptt, stt, _ = ref.elfmm3dtriatargiter(iprec, lam, mu, tri, nrm, src, a, b, tg,
+                                                              icomp_type=1, wsave=w)
+        assert ier == 0
+        out.update({"ptfrc" + tag: pt, "strain" + tag: st, "ptfrctarg" + tag: ptt, "straintarg" + tag: stt})
+    save(name, iprec=iprec, lam=lam, mu=mu, triangle=tri, trinorm=nrm, source=src, sigma_sl=ssl,
+         sigma_dl=sdl, sigma_sl2=ssl2, sigma_dl2=sdl2, target=tg, lused=lused, **out)
+
+
 def main(only=None):
     want = lambda name: only is None or name in only
     tri, nrm, src, ssl, sdl, tg = sphere_case(6, 300, 11)
@@ -98,6 +123,9 @@ def main(only=None):
         fmm_case("fmm_tiny", 2, 2.0, 0.7, t2[:40], n2[:40], s2[:40], sl2[:40], dl2[:40], tg2)
     if want("fmm_fault"):
         fmm_case("fmm_fault", 2, 1.0, 0.8, *fault_case())
+    for name, iprec in (("iter_c1_p2", 2), ("iter_c1_p1", 1)):
+        if want(name):
+            iter_case(name, iprec)
     if only is not None:
         return
 

@@ -22,7 +22,8 @@ def tree_lib(tmp_path_factory):
     out = str(tmp_path_factory.mktemp("tree") / "libtree_check.so")
     subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-o", out,
                     os.path.join(ROOT, "tests", "csrc", "tree_check.cpp"),
-                    os.path.join(CSRC, "elfmm_tree.cpp"), os.path.join(CSRC, "elfmm_pool.cpp")],
+                    os.path.join(CSRC, "elfmm_tree.cpp"), os.path.join(CSRC, "elfmm_plan.cpp"),
+                    os.path.join(CSRC, "elfmm_pool.cpp")],
                    check=True)
     return C.CDLL(out)
 
@@ -95,3 +96,16 @@ def test_tree_matches_oracle(tree_lib, oracle_lib, case):
     lst = {k: np.cumsum([0] + [len(x) for x in o["lists"][k]]).astype(np.int32) for k in range(1, 6)}
     itm = {k: (np.concatenate(o["lists"][k]) if lst[k][-1] else np.zeros(0, np.int32)) for k in range(1, 6)}
     compare_to_reference_layout(t, o["boxes"], o["centers"], o["iz"], o["iztarg"], o["nlev"], lst, itm)
+
+
+@pytest.mark.parametrize("case,nbox", [("iter_c1_p2", 240), ("iter_c1_p1", 120)])
+def test_iter_near_lists_match_reference_store_size(tree_lib, case, nbox):
+    """The iter engine's near lists (self + lists 1, 3, 4, the latter at every
+    point below its box) on the half-space-rooted tree, counted in the
+    reference's store layout, equal the reference's own icomp-4 lused."""
+    g = load_golden(case)
+    src, tgt = g["source"], g["target"]
+    lib_tree(tree_lib, src, tgt, nbox, imindlin=1)
+    tree_lib.plan_iter_ref_words.restype = C.c_double
+    words = tree_lib.plan_iter_ref_words(len(src), 1, 1, 1, 1, 1, 1)
+    assert int(words) == int(g["lused"])

@@ -21,6 +21,14 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64 = 78.6e12        # MI355X FP64 vector / matrix dense peak, FLOP/s
 PEAK_HBM = 8.0e12          # bytes/s
+# operations of one near-field pair (single + double layer, displacement and
+# strain), counted by tests/csrc/nf_opcount.cpp on the library's pair code:
+# 862 adds, 1441 muls, 31 divs, 25 sqrt/log/atan2
+NEAR_OPS_PER_PAIR = 862 + 1441 + 31 + 25
+# HBM bytes per launch of the dominant kernel on this workload, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, scaled by the same counters on a
+# 512 MiB device copy (profiles/near_c2_pmc.md); None until measured
+TRAFFIC = {}
 
 
 def workload(nsplit=64, ntarg=100000, seed=2024):
@@ -136,20 +144,28 @@ def main():
                "strain_rel_l2": float(np.linalg.norm(fs - dst) / np.linalg.norm(dst)),
                "checked_targets": int(len(idx))}
 
-    # dominant kernel and its roofline
+    # dominant kernel and its roofline. Algorithmic work per launch:
+    #  near field: near (point, triangle) pairs x NEAR_OPS_PER_PAIR, the
+    #    counted operations of one pair of the library's pair quadrature
+    #    (tests/csrc/nf_opcount.cpp: adds, muls, divs, 1 per sqrt/log/atan2)
+    #  M2L: per pair and right-hand side the reference's rotate/shift/rotate
+    #    count 2 R(p_t) + Z(p_t) (elfmm_gfx950_last_stats()[5])
     kernels = {k: ph[k] for k in ("p2m", "m2m", "m2l", "p2l", "l2l", "l2t", "m2t", "near")}
+    work = {"near": (stats["near_pairs"] * NEAR_OPS_PER_PAIR, "k_near"),
+            "m2l": (stats["m2l_flops_alg"], "k_gemm (M2L)")}
+    rooflines = {}
+    for k, (alg, name) in work.items():
+        t = kernels[k] / 1e3
+        rooflines[k] = {"kernel": name, "bound": "fp64", "achieved": alg / t / 1e12, "peak": PEAK_FP64 / 1e12,
+                        "unit": "TFLOP/s", "frac": alg / t / PEAK_FP64, "traffic": None,
+                        "algorithmic_flop_per_launch": alg, "ms_per_launch": kernels[k]}
+    rooflines["m2l"]["executed_tflops"] = stats["m2l_flops_dense"] / (kernels["m2l"] / 1e3) / 1e12
     dom = max(kernels, key=kernels.get)
-    roof = None
-    if dom == "m2l":
-        t = ph["m2l"] / 1e3
-        alg = stats["m2l_flops_alg"]
-        roof = {"kernel": "k_gemm (M2L)", "bound": "mfma", "achieved": alg / t / 1e12,
-                "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": alg / t / PEAK_FP64,
-                "traffic": None, "executed_tflops": stats["m2l_flops_dense"] / t / 1e12,
-                "algorithmic_flop_per_launch": alg, "ms_per_launch": ph["m2l"]}
-    else:
-        roof = {"kernel": dom, "bound": "mfma", "achieved": None, "peak": PEAK_FP64 / 1e12,
-                "unit": "TFLOP/s", "frac": None, "traffic": None, "ms_per_launch": kernels[dom]}
+    roof = rooflines.get(dom, {"kernel": dom, "bound": "fp64", "achieved": None, "peak": PEAK_FP64 / 1e12,
+                               "unit": "TFLOP/s", "frac": None, "traffic": None,
+                               "ms_per_launch": kernels[dom]})
+    if TRAFFIC.get(roof["kernel"]) is not None:
+        roof["traffic"] = TRAFFIC[roof["kernel"]]
 
     cpu = None if args.no_cpu else cpu_baseline()
     lat_ms = np.array(lat) * 1e3
@@ -164,7 +180,7 @@ def main():
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
         "accuracy_vs_direct": err,
         "phases_ms": ph, "work": stats,
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "rooflines": rooflines, "cpu_baseline": cpu,
     }
     if cpu:
         line["speedup_vs_cpu_reference"] = value / cpu["value"]

@@ -57,9 +57,12 @@ struct Operators {
     std::vector<double> m2m;             // 8 octants, nc x nc each (row-major)
     std::vector<double> l2l;             // 8 octants
     // M2L per offset (i,j,k) in [-3,3]^3 with max|.|>=2: truncation order
-    // pt and matrix of ncoef(pt)^2 at offset m2l_off[o].
+    // pt, azimuth phi of the translation, and at m2l_off[o] the two real
+    // (ncoef(pt)/2)^2 matrices of the translation rotated into the xz-plane
+    // (real parts, then imaginary parts; see build_operators).
     int m2l_pt[343];
     int64_t m2l_off[343];
+    double m2l_phi[343];
     std::vector<double> m2l;
 };
 void build_operators(int p, const int *itable, Operators &ops);
@@ -73,7 +76,20 @@ struct GemmTile {
     int pair0;       // first pair of this tile
     int npair;       // <= GT_P
 };
-constexpr int GT_M = 64, GT_P = 16, GT_K = 16;
+constexpr int GT_M = 128, GT_P = 32, GT_K = 16;
+
+// One tile of the rotated M2L: ML_M coefficients (real and imaginary
+// parts) of one offset's operator for up to ML_P (source, destination)
+// pairs.
+struct M2LTile {
+    int64_t a_off;   // offset of the real-part matrix (h x h, row-major); imaginary part follows
+    int h;           // coefficients = ncoef(pt) / 2
+    int row0;        // first output coefficient of this tile
+    int pair0;       // first pair of this tile
+    int npair;       // <= ML_P
+    double phi;      // azimuth of the translation
+};
+constexpr int ML_M = 64, ML_P = 32;
 
 struct GemmLaunch {
     std::vector<GemmTile> tiles;
@@ -88,7 +104,8 @@ struct Plan {
     std::vector<double> sides;                       // per box
     std::vector<int> p2m_boxes;                      // leaves forming a multipole
     std::vector<GemmLaunch> m2m_l, l2l_l;            // one launch per level
-    GemmLaunch m2l;                                  // all list-2 pairs, grouped by offset
+    GemmLaunch m2l;                                  // all list-2 pairs, grouped by offset (src/dst/scale)
+    std::vector<M2LTile> m2l_tiles;                  // rotated-M2L tiles over those pairs
     std::vector<int> p2l_boxes, p2l_start, p2l_items;   // box <- sources of its list-4 boxes
     std::vector<int> m2t_boxes, m2t_start, m2t_items;   // box <- multipoles of its list-3 boxes
     std::vector<int> l2t_boxes, nf_boxes, nf_start, nf_items;

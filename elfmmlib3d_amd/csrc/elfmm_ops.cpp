@@ -166,6 +166,13 @@ void build_operators(int p, const int *itable, Operators &ops)
     }
     // M2L per offset; offsets are source ijk minus target ijk, so the
     // translation vector (target centre minus source centre) is -offset.
+    // With t = Rz(phi) t', t' = (rho, 0, tz) in the xz-plane,
+    // I_n^q(t) = e^{i q phi} I_n^q(t') and I_n^q(t') is real, so
+    //   L_j^k = e^{-i k phi} sum_{l,m} a_{jklm}(t') (e^{i m phi} M_l^m)
+    // with real a: on the rotated coefficients the real parts map to real
+    // parts and the imaginary parts to imaginary parts, each by a real
+    // ncoef(pt)/2 square matrix (rows j,k; columns l,m >= 0). Stored per
+    // offset: the real-part matrix, then the imaginary-part matrix.
     size_t tot = 0;
     for (int o = 0; o < 343; o++) {
         int i = o / 49 - 3, j = (o / 7) % 7 - 3, k = o % 7 - 3;
@@ -173,16 +180,18 @@ void build_operators(int p, const int *itable, Operators &ops)
         int pt = far ? std::min(p, itable[o]) : 0;
         ops.m2l_pt[o] = far ? pt : -1;
         ops.m2l_off[o] = (int64_t)tot;
-        if (far) tot += (size_t)ncoef(pt) * ncoef(pt);
+        ops.m2l_phi[o] = far ? std::atan2(-(double)j, -(double)i) : 0.0;
+        if (far) tot += (size_t)2 * (ncoef(pt) / 2) * (ncoef(pt) / 2);
     }
     ops.m2l.assign(tot, 0.0);
     for (int o = 0; o < 343; o++) {
         int pt = ops.m2l_pt[o];
         if (pt < 0) continue;
         int i = o / 49 - 3, j = (o / 7) % 7 - 3, k = o % 7 - 3;
-        int n = ncoef(pt);
-        Harm I = irregular(2 * pt, -(double)i, -(double)j, -(double)k);
-        size_t base = (size_t)ops.m2l_off[o];
+        int n = ncoef(pt), h = n / 2;
+        double rho = std::sqrt((double)(i * i + j * j));
+        Harm I = irregular(2 * pt, rho, 0.0, -(double)k);
+        std::vector<double> D((size_t)n * n, 0.0);
         for (int jo = 0; jo <= pt; jo++)
             for (int ko = 0; ko <= jo; ko++)
                 for (int l = 0; l <= pt; l++)
@@ -190,8 +199,14 @@ void build_operators(int p, const int *itable, Operators &ops)
                         double c = std::sqrt(binom(l + jo - m + ko, jo + ko) *
                                              binom(l + jo + m - ko, jo - ko));
                         double s = ((jo + ko) & 1) ? -1.0 : 1.0;
-                        add_entry(ops.m2l, base, n, jo, ko, l, m, s * c * I.get(l + jo, m - ko));
+                        add_entry(D, 0, n, jo, ko, l, m, s * c * I.get(l + jo, m - ko));
                     }
+        double *Rm = &ops.m2l[(size_t)ops.m2l_off[o]], *Im = Rm + (size_t)h * h;
+        for (int r = 0; r < h; r++)
+            for (int c = 0; c < h; c++) {
+                Rm[(size_t)r * h + c] = D[(size_t)(2 * r) * n + 2 * c];
+                Im[(size_t)r * h + c] = D[(size_t)(2 * r + 1) * n + 2 * c + 1];
+            }
     }
 }
 

@@ -155,25 +155,26 @@ void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool 
         }
         double dense = 0, alg = 0;
         // the pairs are already in offset order; tiles per offset cover
-        // GT_P pairs by GT_M rows of that offset's operator
+        // ML_P pairs by ML_M coefficients of that offset's rotated operator
         std::vector<int64_t> tstart(344, 0);
         for (int o = 0; o < 343; o++) {
             int m = ostart[o + 1] - ostart[o];
-            int dim = ncoef(ops->m2l_pt[o]);
-            tstart[o + 1] = tstart[o] + (m ? (int64_t)((m + GT_P - 1) / GT_P) * ((dim + GT_M - 1) / GT_M) : 0);
+            int h = ncoef(ops->m2l_pt[o]) / 2;
+            tstart[o + 1] = tstart[o] + (m ? (int64_t)((m + ML_P - 1) / ML_P) * ((h + ML_M - 1) / ML_M) : 0);
             if (!m) continue;
-            dense += m * 2.0 * dim * (double)dim * nrhs;
+            // two real h x h products per pair and right-hand side
+            dense += m * 2.0 * 2.0 * h * (double)h * nrhs;
             alg += m * alg_m2l[ops->m2l_pt[o]] * nrhs;
         }
-        P.m2l.tiles.resize(tstart[343]);
+        P.m2l_tiles.resize(tstart[343]);
         parallel_chunks(343, [&](int o, int) {
             int m = ostart[o + 1] - ostart[o];
             if (!m) return;
-            int dim = ncoef(ops->m2l_pt[o]);
-            GemmTile *t = &P.m2l.tiles[tstart[o]];
-            for (int p0 = 0; p0 < m; p0 += GT_P)
-                for (int r0 = 0; r0 < dim; r0 += GT_M)
-                    *t++ = GemmTile{ops->m2l_off[o], dim, r0, ostart[o] + p0, std::min(GT_P, m - p0)};
+            int h = ncoef(ops->m2l_pt[o]) / 2;
+            M2LTile *t = &P.m2l_tiles[tstart[o]];
+            for (int p0 = 0; p0 < m; p0 += ML_P)
+                for (int r0 = 0; r0 < h; r0 += ML_M)
+                    *t++ = M2LTile{ops->m2l_off[o], h, r0, ostart[o] + p0, std::min(ML_P, m - p0), ops->m2l_phi[o]};
         });
         P.m2l.src.swap(ps);
         P.m2l.dst.swap(pd);
@@ -321,7 +322,7 @@ void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool 
         npair_near += npts(P.nf_boxes[q]) * ns;
     }
     P.stats[6] = npair_near;
-    P.stats[7] = (double)P.m2l.tiles.size();
+    P.stats[7] = (double)P.m2l_tiles.size();
     double m2m_pairs = 0, l2l_pairs = 0;
     for (auto &g : P.m2m_l) m2m_pairs += g.src.size();
     for (auto &g : P.l2l_l) l2l_pairs += g.src.size();

@@ -62,10 +62,11 @@ def test_iter_store_and_replay(gpu_lib):
     out1 = run(gpu_lib, g, 1, second=True)
     for a, b in zip(out3[2:], out1[2:]):
         assert rel(a, b) < 1e-13
-    # a second replay gives the same numbers
+    # a second replay gives the same numbers (up to the order of the far
+    # field's atomic additions)
     out3b = run(gpu_lib, g, 3, wsave, second=True)
     for a, b in zip(out3[2:], out3b[2:]):
-        assert np.array_equal(a, b)
+        assert rel(a, b) < 1e-14
     assert gpu_lib.iter_release(wsave) == 1
     assert gpu_lib.iter_release(wsave) == 0
 

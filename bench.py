@@ -25,10 +25,10 @@ PEAK_HBM = 8.0e12          # bytes/s
 # strain), counted by tests/csrc/nf_opcount.cpp on the library's pair code:
 # 862 adds, 1441 muls, 31 divs, 25 sqrt/log/atan2
 NEAR_OPS_PER_PAIR = 862 + 1441 + 31 + 25
-# HBM bytes per launch of the dominant kernel on this workload, from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, scaled by the same counters on a
-# 512 MiB device copy (profiles/near_c2_pmc.md); None until measured
-TRAFFIC = {}
+# HBM bytes per launch on this workload, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE, each scaled by what the counter reads for a 512 MiB device copy
+# in the same run (profiles/c2_apply_kernels.md)
+TRAFFIC = {"k_near": 182.1e6, "k_gemm (M2L)": 10922.1e6}
 
 
 def workload(nsplit=64, ntarg=100000, seed=2024):

@@ -4,9 +4,14 @@ flat triangles (elfmm3dtriatarg), workload C2 of the baseline:
   evaluation at all centroids plus 100,000 targets uniform in [-2,2]^3 minus
   the unit ball, lam = mu = 1, FP64, iprec 2 (tolerance 0.5e-6).
 A step is one complete apply (tree + far field + near field + assembly) with
-inputs already resident in HBM. Prints one JSON line.
+inputs already resident in HBM. The K timed steps are served as a stream of
+requests with --workers applies in flight (library calls on separate host
+threads and streams, so host planning and kernels of different requests
+overlap); value = (triangles + targets) x K / wall time, latency = per
+request from submission to result. The same K applies back to back on one
+stream are reported as single_request. Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--iprec 2] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workers 3] [--iprec 2] [--no-cpu]
 """
 import argparse
 import json
@@ -28,7 +33,7 @@ NEAR_OPS_PER_PAIR = 862 + 1441 + 31 + 25
 # HBM bytes per launch on this workload, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE, each scaled by what the counter reads for a 512 MiB device copy
 # in the same run (profiles/c2_apply_kernels.md)
-TRAFFIC = {"k_near": 182.1e6, "k_gemm (M2L)": 10922.1e6}
+TRAFFIC = {"k_near": 182.1e6, "k_m2l": 10922.1e6}
 
 
 def workload(nsplit=64, ntarg=100000, seed=2024):
@@ -80,6 +85,8 @@ def main():
     ap.add_argument("--ntarg", type=int, default=100000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", type=int, default=2000, help="targets checked against the direct sum")
+    ap.add_argument("--workers", type=int, default=3,
+                    help="requests in flight in the served measurement (1: single-request path only)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="run one device copy of MIB MiB first (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -116,6 +123,7 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    # single-request path: K applies back to back on the default stream
     lat = []
     phases = []
     t0 = time.perf_counter()
@@ -127,10 +135,51 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     stats = E.last_stats()
-    total = t1 - t0
+    single_total = t1 - t0
+    single_value = (n + m) * args.steps / single_total / 1e6
+    ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+    single_lat = np.array(lat) * 1e3
+
+    # served path: the same apply as a stream of requests with `workers` in
+    # flight (each worker a host thread with its own stream); K requests timed
+    served = None
+    if args.workers > 1:
+        from elfmmlib3d_amd.serve import Request, Server
+        req = Request(args.iprec, 1.0, 1.0, d["triangle"], d["trinorm"], d["source"], d["sigma_sl"],
+                      d["sigma_dl"], d["target"])
+        srv = Server(workers=args.workers)
+        for f in [srv.submit(req) for _ in range(max(args.warmup, args.workers))]:
+            assert f.result().ier == 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pending, results, submitted = [], [], 0
+        while submitted < min(args.workers, args.steps):
+            pending.append(srv.submit(req))
+            submitted += 1
+        while pending:
+            r = pending.pop(0).result()
+            results.append(r)
+            if submitted < args.steps:
+                pending.append(srv.submit(req))
+                submitted += 1
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        srv.close()
+        assert all(r.ier == 0 for r in results)
+        last = results[-1].numpy()
+        diff = max(float(np.linalg.norm(a - b.cpu().numpy()) / np.linalg.norm(b.cpu().numpy()))
+                   for a, b in zip(last, (out["ptfrc"], out["strain"], out["ptfrctarg"], out["straintarg"])))
+        served = {"total_s": t1 - t0, "lat_ms": np.array([r.latency_ms for r in results]),
+                  "vs_single_rel": diff}
+
+    if served:
+        total = served["total_s"]
+        lat_ms = served["lat_ms"]
+    else:
+        total = single_total
+        lat_ms = single_lat
     ms_per_step = 1e3 * total / args.steps
     value = (n + m) * args.steps / total / 1e6
-    ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
 
     # accuracy against the direct sum at sampled targets
     err = {}
@@ -152,7 +201,7 @@ def main():
     #    count 2 R(p_t) + Z(p_t) (elfmm_gfx950_last_stats()[5])
     kernels = {k: ph[k] for k in ("p2m", "m2m", "m2l", "p2l", "l2l", "l2t", "m2t", "near")}
     work = {"near": (stats["near_pairs"] * NEAR_OPS_PER_PAIR, "k_near"),
-            "m2l": (stats["m2l_flops_alg"], "k_gemm (M2L)")}
+            "m2l": (stats["m2l_flops_alg"], "k_m2l")}
     rooflines = {}
     for k, (alg, name) in work.items():
         t = kernels[k] / 1e3
@@ -168,7 +217,6 @@ def main():
         roof["traffic"] = TRAFFIC[roof["kernel"]]
 
     cpu = None if args.no_cpu else cpu_baseline()
-    lat_ms = np.array(lat) * 1e3
     line = {
         "metric": "FMM apply throughput (M tris+targets/s) @1e-6 err; p50/p99 served latency",
         "value": value, "unit": "M tris+targets/s", "n_gpus": 1, "steps": args.steps,
@@ -178,6 +226,11 @@ def main():
                                "targets, lam=mu=1", "iprec": args.iprec, "triangles": n, "targets": m,
                    "tolerance": {1: 0.5e-3, 2: 0.5e-6, 3: 0.5e-9}.get(args.iprec)},
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+        "served": ({"workers": args.workers, "requests": args.steps,
+                    "result_vs_single_request_rel": served["vs_single_rel"]} if served else None),
+        "single_request": {"value": single_value, "ms_per_step": 1e3 * single_total / args.steps,
+                           "latency_ms": {"p50": float(np.percentile(single_lat, 50)),
+                                          "p99": float(np.percentile(single_lat, 99))}},
         "accuracy_vs_direct": err,
         "phases_ms": ph, "work": stats,
         "roofline": roof, "rooflines": rooflines, "cpu_baseline": cpu,

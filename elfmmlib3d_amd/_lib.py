@@ -32,6 +32,7 @@ def library():
             pass
         _lib = C.CDLL(LIB)
         _lib.elfmm_gfx950_triatarg_dev.restype = C.c_int
+        _lib.elfmm_gfx950_triatargiter_dev.restype = C.c_int
         _lib.elfmm_gfx950_version.restype = C.c_char_p
     return _lib
 

@@ -1,0 +1,188 @@
+"""Serving: continuous batching of FMM applies on one GPU.
+
+Requests (one elfmm3dtriatarg or elfmm3dtriatargiter apply each, of any
+size) are admitted while a worker slot and device-memory budget are free;
+each worker is a host thread with its own HIP stream and its own library
+workspace, so one request's host planning (octree, lists, task lists)
+overlaps other requests' kernels, and kernels of different requests share
+the GPU. A request's result does not depend on what runs beside it (the
+library keeps no shared mutable state between applies except the cached
+translation operators and the iter stores a request names itself).
+
+    srv = Server(workers=3)
+    fut = srv.submit(Request(iprec=2, rlam=1.0, rmu=1.0, triangle=..., ...))
+    res = fut.result()          # Result(ier, ptfrc, strain, ptfrctarg, straintarg, latency_ms, ...)
+    srv.close()
+
+Inputs may be numpy arrays (host) or float64 torch tensors on the GPU;
+outputs are torch tensors on the GPU (``Result.numpy()`` copies them out).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import queue
+import threading
+import time
+from concurrent.futures import Future
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+
+@dataclass
+class Request:
+    iprec: int
+    rlam: float
+    rmu: float
+    triangle: object
+    trinorm: object
+    source: object
+    sigma_sl: object = None
+    sigma_dl: object = None
+    target: object = None
+    ifptfrc: int = 1
+    ifstrain: int = 1
+    ifptfrctarg: int = 1
+    ifstraintarg: int = 1
+    # iter engine: icomp_type 1/2/3/5 with a caller-owned wsave (numpy,
+    # >= 4 words) that names the device-resident near-field store
+    icomp_type: Optional[int] = None
+    wsave: Optional[np.ndarray] = None
+
+
+@dataclass
+class Result:
+    ier: int
+    ptfrc: object
+    strain: object
+    ptfrctarg: object
+    straintarg: object
+    latency_ms: float
+    queue_ms: float
+    timings: dict = field(default_factory=dict)
+
+    def numpy(self):
+        f = lambda t: None if t is None else t.cpu().numpy()
+        return f(self.ptfrc), f(self.strain), f(self.ptfrctarg), f(self.straintarg)
+
+
+def estimate_bytes(n, nt, p=26):
+    """Device memory of one apply's workspace (expansions, nodes, fields,
+    records, inputs/outputs), a conservative estimate used for admission."""
+    nboxes = 2 * (n + nt) // 40 + 64
+    nc = (p + 1) * (p + 2)
+    exps = 2 * nboxes * 4 * nc * 8
+    fields = nboxes * 4 * 9 * (p + 3) * (p + 4) * 8
+    pointwise = (n + nt) * (40 + 9 + 3 + 24) * 8 + n * (12 * 20 + 18 + 24) * 8
+    io = (n * 27 + nt * 15) * 8
+    return int(exps + fields + pointwise + io) + (64 << 20)
+
+
+class Server:
+    def __init__(self, workers=3, hbm_budget=None, device=0):
+        import torch
+        from . import _lib
+        self._torch = torch
+        self._lib = _lib
+        _lib.library()
+        self.device = torch.device("cuda", device)
+        if hbm_budget is None:
+            free, _ = torch.cuda.mem_get_info(self.device)
+            hbm_budget = int(free * 0.8)
+        self.budget = hbm_budget
+        self._inflight = 0
+        self._cv = threading.Condition()
+        self._q: "queue.Queue" = queue.Queue()
+        self._closed = False
+        self._threads = [threading.Thread(target=self._worker, daemon=True) for _ in range(workers)]
+        for t in self._threads:
+            t.start()
+
+    def submit(self, req: Request) -> Future:
+        fut: Future = Future()
+        self._q.put((req, fut, time.perf_counter()))
+        return fut
+
+    def close(self):
+        self._closed = True
+        for _ in self._threads:
+            self._q.put(None)
+        for t in self._threads:
+            t.join()
+
+    # -- worker side ------------------------------------------------------
+    def _admit(self, need):
+        with self._cv:
+            # a request larger than the whole budget runs alone
+            self._cv.wait_for(lambda: self._inflight == 0 or self._inflight + need <= self.budget)
+            self._inflight += need
+
+    def _release(self, need):
+        with self._cv:
+            self._inflight -= need
+            self._cv.notify_all()
+
+    def _worker(self):
+        torch = self._torch
+        torch.cuda.set_device(self.device)
+        stream = torch.cuda.Stream(device=self.device)
+        while True:
+            item = self._q.get()
+            if item is None:
+                return
+            req, fut, t_submit = item
+            n = req.triangle.shape[0]
+            nt = 0 if req.target is None else req.target.shape[0]
+            need = estimate_bytes(n, nt)
+            self._admit(need)
+            t_start = time.perf_counter()
+            try:
+                res = self._run(req, stream)
+                res.queue_ms = (t_start - t_submit) * 1e3
+                res.latency_ms = (time.perf_counter() - t_submit) * 1e3
+                fut.set_result(res)
+            except Exception as e:                       # noqa: BLE001 (reported to the caller)
+                fut.set_exception(e)
+            finally:
+                self._release(need)
+
+    def _run(self, req: Request, stream):
+        torch = self._torch
+        dev = self.device
+        lib = self._lib.library()
+
+        def dt(a, shape):
+            if a is None:
+                return None
+            if isinstance(a, torch.Tensor):
+                return a.to(device=dev, dtype=torch.float64).contiguous().reshape(shape)
+            return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64).reshape(shape)).to(
+                dev, non_blocking=False)
+
+        n = req.triangle.shape[0]
+        nt = 0 if req.target is None else req.target.shape[0]
+        with torch.cuda.stream(stream):
+            tri, nrm, src = dt(req.triangle, (n, 3, 3)), dt(req.trinorm, (n, 3)), dt(req.source, (n, 3))
+            ssl, sdl = dt(req.sigma_sl, (n, 3)), dt(req.sigma_dl, (n, 3))
+            tgt = dt(req.target, (nt, 3)) if nt else None
+            z = lambda shape, flag: torch.zeros(shape, dtype=torch.float64, device=dev) if flag else None
+            pt, st = z((n, 3), req.ifptfrc), z((n, 3, 3), req.ifstrain)
+            ptt, stt = z((nt, 3), req.ifptfrctarg and nt), z((nt, 3, 3), req.ifstraintarg and nt)
+            stream.synchronize()
+        ptr = lambda t: C.c_void_p(0 if t is None else t.data_ptr())
+        s = C.c_void_p(stream.cuda_stream)
+        args = [C.c_int(req.iprec), C.c_double(req.rlam), C.c_double(req.rmu), C.c_int(n), ptr(tri), ptr(nrm),
+                ptr(src), C.c_int(ssl is not None), ptr(ssl), C.c_int(sdl is not None), ptr(sdl),
+                C.c_int(pt is not None), ptr(pt), C.c_int(st is not None), ptr(st), C.c_int(nt), ptr(tgt),
+                C.c_int(ptt is not None), ptr(ptt), C.c_int(stt is not None), ptr(stt)]
+        if req.icomp_type is None:
+            ier = lib.elfmm_gfx950_triatarg_dev(*args, s)
+        else:
+            ws = req.wsave if req.wsave is not None else np.zeros(4)
+            lused = C.c_int(0)
+            ier = lib.elfmm_gfx950_triatargiter_dev(*args, C.c_int(req.icomp_type),
+                                                    ws.ctypes.data_as(C.c_void_p), C.c_int(ws.size),
+                                                    C.byref(lused), s)
+        timings = self._lib.last_timings()           # same host thread: this request's phases
+        return Result(ier, pt, st, ptt, stt, 0.0, 0.0, timings)

@@ -67,6 +67,21 @@ struct Operators {
 };
 void build_operators(int p, const int *itable, Operators &ops);
 
+// Gather table of the field operand of L2T (irr = false, harmonics to
+// degree p) and M2T (irr = true, degree p + 2). Entry (pos*2 + part)*9 + q
+// gives the real (part 0) or minus the imaginary (part 1) part of field q
+// (0 value, 1-3 d/dx d/dy d/dz, 4 xx, 5 yy, 6 xy, 7 xz, 8 yz) of one
+// packed real expansion of order p, at the harmonic with stream position
+// pos (m outer, l inner; padded to an even count with zero entries), times
+// the weight of the real field sum (1 for m = 0, else 2) and
+// 1/sqrt((l+m)!(l-m)!): the sum of c[t] * E[idx[t]], t < 3 (E as doubles,
+// re/im interleaved). Every such entry has at most three terms.
+struct FieldTerm {
+    int idx[4];
+    double c[4];
+};
+void build_field_table(int p, bool irr, std::vector<FieldTerm> &tab);
+
 // One tile of a dense translation launch: GT_M output rows of one operator
 // matrix applied to up to GT_P (source, destination) pairs.
 struct GemmTile {
@@ -109,10 +124,15 @@ struct Plan {
     std::vector<int> p2l_boxes, p2l_start, p2l_items;   // box <- sources of its list-4 boxes
     std::vector<int> m2t_boxes, m2t_start, m2t_items;   // box <- multipoles of its list-3 boxes
     std::vector<int> l2t_boxes, nf_boxes, nf_start, nf_items;
-    std::vector<int> l2t_tpt, l2t_tslot;             // (point, l2t slot) tasks
-    std::vector<int> m2t_slot_box, m2t_tpt, m2t_tslot;   // (point, m2t slot) tasks
+    std::vector<int> l2t_tpt;                        // points of each l2t slot, slot after slot
+    std::vector<int> m2t_slot_box, m2t_tpt;          // points of each m2t slot, slot after slot
+    // evaluation wave tasks: (slot, first task, task count <= EW_PTS) triples
+    // over the slot-contiguous (point, slot) tasks above
+    std::vector<int> l2t_wt, m2t_wt;
     double stats[16];                                // as elfmm_gfx950_last_stats
 };
+
+constexpr int EW_PTS = 32;                           // points per evaluation wave task
 
 // Interaction plan of one apply (the work of the reference's
 // lfmm3dtrilhesstargmain steps 1-7 and of the near-field pass, as task

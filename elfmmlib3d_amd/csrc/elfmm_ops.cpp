@@ -23,6 +23,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <stdexcept>
 #include <vector>
 
 #include "elfmm_internal.h"
@@ -208,6 +209,157 @@ void build_operators(int p, const int *itable, Operators &ops)
                 Im[(size_t)r * h + c] = D[(size_t)(2 * r + 1) * n + 2 * c + 1];
             }
     }
+}
+
+namespace {
+
+// Real linear form over the doubles of a packed expansion.
+struct LForm {
+    std::vector<std::pair<int, double>> t;
+    void add(const LForm &o, double s)
+    {
+        for (auto &e : o.t) t.push_back({e.first, e.second * s});
+    }
+};
+struct CForm {
+    LForm re, im;
+};
+
+CForm cscale(const CForm &a, double s)
+{
+    CForm r;
+    r.re.add(a.re, s);
+    r.im.add(a.im, s);
+    return r;
+}
+
+// coefficient (l, m) of a real-field expansion of order pe whose m >= 0
+// coefficients are the forms f(l, m): c^{-m} = (-1)^m conj(c^m), zero
+// outside 0 <= |m| <= l <= pe
+template <class F>
+CForm sym_get(int pe, int l, int m, F &&f)
+{
+    int am = std::abs(m);
+    if (l < 0 || l > pe || am > l) return CForm();
+    CForm c = f(l, am);
+    if (m < 0) {
+        double sr = (am & 1) ? -1.0 : 1.0;
+        CForm r;
+        r.re.add(c.re, sr);
+        r.im.add(c.im, -sr);
+        return r;
+    }
+    return c;
+}
+
+// gradient (x, y, z) at (n, k) from the coefficients c (order k), a (k-1)
+// and b (k+1) at degree n+1 (regular) or n-1 (irregular)
+void grad_comb(bool irr, int n, int k, const CForm &c, const CForm &a, const CForm &b, CForm g[3])
+{
+    auto sq = [](int v) { return std::sqrt((double)v); };
+    double fz, fa, fb;
+    if (!irr) {
+        fz = sq((n + 1 - k) * (n + 1 + k));
+        fa = sq((n + 2 - k) * (n + 1 - k));
+        fb = -sq((n + 2 + k) * (n + 1 + k));
+    } else {
+        fz = -sq((n - k) * (n + k));
+        fa = sq(std::max(n + k - 1, 0) * (n + k));
+        fb = -sq(std::max(n - k - 1, 0) * (n - k));
+    }
+    g[2] = cscale(c, fz);
+    CForm A = cscale(a, fa), B = cscale(b, fb);
+    g[0] = CForm();
+    g[0].re.add(A.re, 0.5);
+    g[0].re.add(B.re, 0.5);
+    g[0].im.add(A.im, 0.5);
+    g[0].im.add(B.im, 0.5);
+    // y = (A - B) / (2i)
+    g[1] = CForm();
+    g[1].re.add(A.im, 0.5);
+    g[1].re.add(B.im, -0.5);
+    g[1].im.add(A.re, -0.5);
+    g[1].im.add(B.re, 0.5);
+}
+
+}  // namespace
+
+void build_field_table(int p, bool irr, std::vector<FieldTerm> &tab)
+{
+    int ph = irr ? p + 2 : p, p1 = irr ? p + 1 : p - 1, p2 = irr ? p + 2 : p - 2;
+    int ncf = (ph + 1) * (ph + 2) / 2;
+    int npos = ncf + (ncf & 1);
+    tab.assign((size_t)npos * 2 * 9, FieldTerm{{0, 0, 0, 0}, {0.0, 0.0, 0.0, 0.0}});
+    std::vector<double> fact(2 * ph + 2, 1.0);
+    for (size_t k = 1; k < fact.size(); k++) fact[k] = fact[k - 1] * (double)k;
+    auto e0 = [&](int l, int m) {
+        CForm c;
+        int j = 2 * (l * (l + 1) / 2 + m);
+        c.re.t.push_back({j, 1.0});
+        c.im.t.push_back({j + 1, 1.0});
+        return c;
+    };
+    auto E = [&](int l, int m) { return sym_get(p, l, m, e0); };
+    // gradient of the expansion at (l, m >= 0)
+    auto grad = [&](int l, int m, CForm g[3]) {
+        int d = irr ? l - 1 : l + 1;
+        grad_comb(irr, l, m, E(d, m), E(d, m - 1), E(d, m + 1), g);
+    };
+    for (int m = 0; m <= ph; m++)
+        for (int l = m; l <= ph; l++) {
+            int pos = m * (ph + 1) - m * (m - 1) / 2 + (l - m);
+            CForm f[9];
+            if (l <= p) f[0] = E(l, m);
+            if (l <= p1) grad(l, m, &f[1]);
+            if (l <= p2) {
+                int d = irr ? l - 1 : l + 1;
+                CForm g[3][3];     // gradient at (d, m), (d, m-1), (d, m+1)
+                for (int u = 0; u < 3; u++) {
+                    int mm = u == 0 ? m : (u == 1 ? m - 1 : m + 1);
+                    for (int v = 0; v < 3; v++) {
+                        g[u][v] = sym_get(p1, d, mm, [&](int ll, int am) {
+                            CForm gg[3];
+                            grad(ll, am, gg);
+                            return gg[v];
+                        });
+                    }
+                }
+                CForm hx[3], hy[3];
+                grad_comb(irr, l, m, g[0][0], g[1][0], g[2][0], hx);   // of d/dx: xx, xy, xz
+                grad_comb(irr, l, m, g[0][1], g[1][1], g[2][1], hy);   // of d/dy: yx, yy, yz
+                f[4] = hx[0];
+                f[6] = hx[1];
+                f[7] = hx[2];
+                f[5] = hy[1];
+                f[8] = hy[2];
+            }
+            double w = (m == 0 ? 1.0 : 2.0) / std::sqrt(fact[l + m] * fact[l - m]);
+            for (int q = 0; q < 9; q++)
+                for (int part = 0; part < 2; part++) {
+                    const LForm &lf = part ? f[q].im : f[q].re;
+                    double sg = part ? -w : w;
+                    // merge equal indices, drop exact zeros
+                    std::vector<std::pair<int, double>> mt;
+                    for (auto &e : lf.t) {
+                        bool found = false;
+                        for (auto &x : mt)
+                            if (x.first == e.first) {
+                                x.second += e.second;
+                                found = true;
+                            }
+                        if (!found) mt.push_back(e);
+                    }
+                    FieldTerm &ft = tab[((size_t)pos * 2 + part) * 9 + q];
+                    int nt = 0;
+                    for (auto &x : mt) {
+                        if (x.second == 0.0) continue;
+                        if (nt == 3) throw std::runtime_error("field table: more than three terms");
+                        ft.idx[nt] = x.first;
+                        ft.c[nt] = x.second * sg;
+                        nt++;
+                    }
+                }
+        }
 }
 
 }  // namespace elfmm

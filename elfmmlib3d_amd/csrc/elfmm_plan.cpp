@@ -70,6 +70,20 @@ GemmLaunch tree_pass(const Tree &T, int lev, int nc, bool up)
 
 constexpr int CHUNK = 64;
 
+// Split each slot's task range [off[s], off[s+1]) into evaluation wave tasks
+// of at most EW_PTS points: (slot, first task, count) triples.
+template <class I>
+void wave_tasks(const std::vector<I> &off, std::vector<int> &wt)
+{
+    wt.clear();
+    for (size_t s = 0; s + 1 < off.size(); s++)
+        for (I t = off[s]; t < off[s + 1]; t += EW_PTS) {
+            wt.push_back((int)s);
+            wt.push_back((int)t);
+            wt.push_back((int)std::min<I>(EW_PTS, off[s + 1] - t));
+        }
+}
+
 }  // namespace
 
 void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool direct_only, int nrhs,
@@ -248,29 +262,23 @@ void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool 
     if (direct_only) P.nf_boxes.clear();
 
     // evaluation tasks: L2T one per point of each leaf; M2T one per (point,
-    // list-3 box), grouped by the box whose derivative fields are read
-    auto fill_points = [&](int b, int slot, int *tp, int *ts) {
+    // list-3 box), grouped by the box whose field operand is read
+    auto fill_points = [&](int b, int *tp) {
         int k = 0;
         if (src_eval)
-            for (int i = T.src_start[b]; i < T.src_start[b] + T.src_count[b]; i++, k++) {
-                tp[k] = i;
-                ts[k] = slot;
-            }
-        for (int i = T.tgt_start[b]; i < T.tgt_start[b] + T.tgt_count[b]; i++, k++) {
-            tp[k] = n + i;
-            ts[k] = slot;
-        }
+            for (int i = T.src_start[b]; i < T.src_start[b] + T.src_count[b]; i++, k++) tp[k] = i;
+        for (int i = T.tgt_start[b]; i < T.tgt_start[b] + T.tgt_count[b]; i++, k++) tp[k] = n + i;
     };
     {
         int nl = (int)P.l2t_boxes.size();
         std::vector<int> off(nl + 1, 0);
         for (int q = 0; q < nl; q++) off[q + 1] = off[q] + npts(P.l2t_boxes[q]);
         P.l2t_tpt.resize(off[nl]);
-        P.l2t_tslot.resize(off[nl]);
         parallel_chunks((nl + CHUNK - 1) / CHUNK, [&](int ch, int) {
             for (int q = ch * CHUNK; q < std::min(nl, (ch + 1) * CHUNK); q++)
-                fill_points(P.l2t_boxes[q], q, &P.l2t_tpt[off[q]], &P.l2t_tslot[off[q]]);
+                fill_points(P.l2t_boxes[q], &P.l2t_tpt[off[q]]);
         });
+        wave_tasks(off, P.l2t_wt);
     }
     if (!P.m2t_items.empty()) {
         // slots in order of first use; the owners of each slot in order
@@ -299,16 +307,16 @@ void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool 
             toff[sl + 1] = toff[sl] + t;
         }
         P.m2t_tpt.resize(toff[ns]);
-        P.m2t_tslot.resize(toff[ns]);
         parallel_chunks((ns + CHUNK - 1) / CHUNK, [&](int ch, int) {
             for (int sl = ch * CHUNK; sl < std::min(ns, (ch + 1) * CHUNK); sl++) {
                 int64_t at = toff[sl];
                 for (int k = ostart[sl]; k < ostart[sl + 1]; k++) {
-                    fill_points(owners[k], sl, &P.m2t_tpt[at], &P.m2t_tslot[at]);
+                    fill_points(owners[k], &P.m2t_tpt[at]);
                     at += npts(owners[k]);
                 }
             }
         });
+        wave_tasks(toff, P.m2t_wt);
     }
 
     // remaining work counts (elfmm_gfx950_last_stats)

@@ -228,6 +228,26 @@ void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool 
                 P.m2t_start.push_back((int)P.m2t_items.size());
             }
         }
+        // P2L boxes in decreasing order of source count: one workgroup per
+        // box, so the longest ones start first and the launch ends evenly
+        int n4 = (int)P.p2l_boxes.size();
+        std::vector<int64_t> work(n4, 0);
+        for (int q = 0; q < n4; q++)
+            for (int k = P.p2l_start[q]; k < P.p2l_start[q + 1]; k++) work[q] += T.src_count[P.p2l_items[k]];
+        std::vector<int> ord(n4);
+        for (int q = 0; q < n4; q++) ord[q] = q;
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return work[a] > work[b]; });
+        std::vector<int> bx(n4), st(1, 0), it;
+        it.reserve(P.p2l_items.size());
+        for (int q = 0; q < n4; q++) {
+            int o = ord[q];
+            bx[q] = P.p2l_boxes[o];
+            it.insert(it.end(), P.p2l_items.begin() + P.p2l_start[o], P.p2l_items.begin() + P.p2l_start[o + 1]);
+            st.push_back((int)it.size());
+        }
+        P.p2l_boxes.swap(bx);
+        P.p2l_start.swap(st);
+        P.p2l_items.swap(it);
     }
 
     // leaves: L2T (level >= 2) and near field over self + list 1. With

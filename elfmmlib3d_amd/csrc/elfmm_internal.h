@@ -129,6 +129,9 @@ struct Plan {
     // evaluation wave tasks: (slot, first task, task count <= EW_PTS) triples
     // over the slot-contiguous (point, slot) tasks above
     std::vector<int> l2t_wt, m2t_wt;
+    // M2T workgroups: first wave task of each (all of one slot, at most 4),
+    // then the total
+    std::vector<int> m2t_bt;
     double stats[16];                                // as elfmm_gfx950_last_stats
 };
 

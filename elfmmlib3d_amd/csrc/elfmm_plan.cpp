@@ -337,6 +337,11 @@ void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool 
             }
         });
         wave_tasks(toff, P.m2t_wt);
+        // workgroups of up to 4 consecutive wave tasks of one slot
+        int nw = (int)P.m2t_wt.size() / 3;
+        for (int w = 0; w < nw; w++)
+            if (w == 0 || P.m2t_wt[3 * w] != P.m2t_wt[3 * (w - 1)] || w - P.m2t_bt.back() == 4) P.m2t_bt.push_back(w);
+        P.m2t_bt.push_back(nw);
     }
 
     // remaining work counts (elfmm_gfx950_last_stats)

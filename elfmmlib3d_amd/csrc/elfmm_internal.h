@@ -54,8 +54,11 @@ inline int ncoef(int p) { return (p + 1) * (p + 2); }
 struct Operators {
     int p = 0;
     int nc = 0;                          // ncoef(p)
-    std::vector<double> m2m;             // 8 octants, nc x nc each (row-major)
-    std::vector<double> l2l;             // 8 octants
+    // M2M / L2L per octant, rotated into the xz-plane (as M2L): two real
+    // (nc/2)^2 matrices each, real-part then imaginary-part, and the azimuth
+    std::vector<double> m2m;
+    std::vector<double> l2l;
+    double oct_phi[8];
     // M2L per offset (i,j,k) in [-3,3]^3 with max|.|>=2: truncation order
     // pt, azimuth phi of the translation, and at m2l_off[o] the two real
     // (ncoef(pt)/2)^2 matrices of the translation rotated into the xz-plane
@@ -82,20 +85,9 @@ struct FieldTerm {
 };
 void build_field_table(int p, bool irr, std::vector<FieldTerm> &tab);
 
-// One tile of a dense translation launch: GT_M output rows of one operator
-// matrix applied to up to GT_P (source, destination) pairs.
-struct GemmTile {
-    int64_t a_off;   // matrix offset (doubles), row-major dim x dim
-    int dim;         // rows = cols
-    int row0;        // first output row of this tile
-    int pair0;       // first pair of this tile
-    int npair;       // <= GT_P
-};
-constexpr int GT_M = 128, GT_P = 32, GT_K = 16;
-
-// One tile of the rotated M2L: ML_M coefficients (real and imaginary
-// parts) of one offset's operator for up to ML_P (source, destination)
-// pairs.
+// One tile of a rotated translation (M2L, M2M, L2L): ML_M coefficients
+// (real and imaginary parts) of one operator for up to ML_P (source,
+// destination) pairs.
 struct M2LTile {
     int64_t a_off;   // offset of the real-part matrix (h x h, row-major); imaginary part follows
     int h;           // coefficients = ncoef(pt) / 2
@@ -106,8 +98,10 @@ struct M2LTile {
 };
 constexpr int ML_M = 64, ML_P = 32;
 
+// One launch of rotated translations (M2M or L2L of one level): pairs and
+// their tiles, run by the same kernel as M2L.
 struct GemmLaunch {
-    std::vector<GemmTile> tiles;
+    std::vector<M2LTile> tiles;
     std::vector<int> src, dst;
     std::vector<double> scale;
     size_t tile_off = 0, off_src = 0, off_dst = 0, off_sc = 0;   // offsets in the device plan buffer

@@ -22,6 +22,7 @@
 //                    R_j^k(r) I_{l+j}^{m-k}(a)        (|r| < |a|).
 #include <cmath>
 #include <complex>
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
 #include <vector>
@@ -131,17 +132,26 @@ void add_entry(std::vector<double> &A, size_t base, int ncol, int jo, int ko, in
 void build_operators(int p, const int *itable, Operators &ops)
 {
     ops.p = p;
-    int nc = ncoef(p);
+    int nc = ncoef(p), h = nc / 2;
     ops.nc = nc;
-    ops.m2m.assign((size_t)8 * nc * nc, 0.0);
-    ops.l2l.assign((size_t)8 * nc * nc, 0.0);
+    // M2M / L2L per octant, rotated like M2L below: the child-centre offset
+    // t = Rz(phi) t' with t' = (rho, 0, tz) in the xz-plane, where the
+    // regular harmonics R_n^q(t') are real, so on coefficients rotated by
+    // e^{i m phi} the translation is real: real parts to real parts and
+    // imaginary parts to imaginary parts, two h x h matrices per octant
+    // (real-part matrix, then imaginary-part matrix).
+    ops.m2m.assign((size_t)8 * 2 * h * h, 0.0);
+    ops.l2l.assign((size_t)8 * 2 * h * h, 0.0);
     std::vector<double> pw(p + 1);
     for (int l = 0; l <= p; l++) pw[l] = std::ldexp(1.0, -l);
+    std::vector<double> Dm((size_t)nc * nc), Dl((size_t)nc * nc);
     for (int o = 0; o < 8; o++) {
         // child centre minus parent centre, in units of the parent side
         double t[3] = {(o & 1) ? 0.25 : -0.25, (o & 2) ? 0.25 : -0.25, (o & 4) ? 0.25 : -0.25};
-        Harm R = regular(p, t[0], t[1], t[2]);
-        size_t base = (size_t)o * nc * nc;
+        ops.oct_phi[o] = std::atan2(t[1], t[0]);
+        Harm R = regular(p, std::sqrt(t[0] * t[0] + t[1] * t[1]), 0.0, t[2]);
+        std::fill(Dm.begin(), Dm.end(), 0.0);
+        std::fill(Dl.begin(), Dl.end(), 0.0);
         for (int l = 0; l <= p; l++)
             for (int m = 0; m <= l; m++)
                 for (int j = 0; j <= l; j++)
@@ -149,11 +159,9 @@ void build_operators(int p, const int *itable, Operators &ops)
                         if (std::abs(m - k) > l - j) continue;
                         double c = std::sqrt(binom(l + m, j + k) * binom(l - m, j - k));
                         // M2M: Mp_l^m += c 2^-j Mc_j^k conj(R_{l-j}^{m-k}(t))
-                        add_entry(ops.m2m, base, nc, l, m, j, k,
-                                  c * pw[j] * std::conj(R.get(l - j, m - k)));
+                        add_entry(Dm, 0, nc, l, m, j, k, c * pw[j] * std::conj(R.get(l - j, m - k)));
                         // L2L: Lc_j^k += c 2^-j Lp_l^m R_{l-j}^{m-k}(t)
-                        if (k >= 0)
-                            add_entry(ops.l2l, base, nc, j, k, l, m, c * pw[j] * R.get(l - j, m - k));
+                        if (k >= 0) add_entry(Dl, 0, nc, j, k, l, m, c * pw[j] * R.get(l - j, m - k));
                     }
         // L2L needs inputs of negative order too
         for (int l = 0; l <= p; l++)
@@ -162,8 +170,22 @@ void build_operators(int p, const int *itable, Operators &ops)
                     for (int k = 0; k <= j; k++) {
                         if (std::abs(m - k) > l - j) continue;
                         double c = std::sqrt(binom(l + m, j + k) * binom(l - m, j - k));
-                        add_entry(ops.l2l, base, nc, j, k, l, m, c * pw[j] * R.get(l - j, m - k));
+                        add_entry(Dl, 0, nc, j, k, l, m, c * pw[j] * R.get(l - j, m - k));
                     }
+        for (int which = 0; which < 2; which++) {
+            const std::vector<double> &D = which ? Dl : Dm;
+            double *Rm = (which ? ops.l2l : ops.m2m).data() + (size_t)o * 2 * h * h, *Im = Rm + (size_t)h * h;
+            double big = 0, cross = 0;
+            for (int r = 0; r < h; r++)
+                for (int c = 0; c < h; c++) {
+                    Rm[(size_t)r * h + c] = D[(size_t)(2 * r) * nc + 2 * c];
+                    Im[(size_t)r * h + c] = D[(size_t)(2 * r + 1) * nc + 2 * c + 1];
+                    big = std::max(big, std::max(std::abs(Rm[(size_t)r * h + c]), std::abs(Im[(size_t)r * h + c])));
+                    cross = std::max(cross, std::max(std::abs(D[(size_t)(2 * r) * nc + 2 * c + 1]),
+                                                     std::abs(D[(size_t)(2 * r + 1) * nc + 2 * c])));
+                }
+            if (cross > 1e-13 * big) throw std::runtime_error("rotated M2M/L2L operator is not real");
+        }
     }
     // M2L per offset; offsets are source ijk minus target ijk, so the
     // translation vector (target centre minus source centre) is -offset.

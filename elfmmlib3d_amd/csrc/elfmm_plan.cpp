@@ -18,16 +18,16 @@
 namespace elfmm {
 namespace {
 
-void add_pairs(GemmLaunch &g, int64_t a_off, int dim, const int *src, const int *dst, const double *scale,
+void add_pairs(GemmLaunch &g, int64_t a_off, int h, double phi, const int *src, const int *dst, const double *scale,
                int np)
 {
     int base = (int)g.src.size();
     g.src.insert(g.src.end(), src, src + np);
     g.dst.insert(g.dst.end(), dst, dst + np);
     g.scale.insert(g.scale.end(), scale, scale + np);
-    for (int p0 = 0; p0 < np; p0 += GT_P)
-        for (int r0 = 0; r0 < dim; r0 += GT_M)
-            g.tiles.push_back({a_off, dim, r0, base + p0, std::min(GT_P, np - p0)});
+    for (int p0 = 0; p0 < np; p0 += ML_P)
+        for (int r0 = 0; r0 < h; r0 += ML_M)
+            g.tiles.push_back({a_off, h, r0, base + p0, std::min(ML_P, np - p0), phi});
 }
 
 // octant of child c in parent b: bit d set when c is the upper half in d
@@ -45,9 +45,11 @@ inline int offset_index(const Tree &T, int j, int b)
            (T.ijk[3 * j + 2] - T.ijk[3 * b + 2] + 3);
 }
 
-// M2M (up) or L2L (down) launch for one level: pairs grouped by octant.
-GemmLaunch tree_pass(const Tree &T, int lev, int nc, bool up)
+// M2M (up) or L2L (down) launch for one level: pairs grouped by octant,
+// each octant's rotated operator (two h x h matrices, h = nc / 2).
+GemmLaunch tree_pass(const Tree &T, int lev, int nc, const double *oct_phi, bool up)
 {
+    int h = nc / 2;
     GemmLaunch g;
     std::vector<int> s[8], d[8];
     for (int b = T.laddr[2 * lev]; b < T.laddr[2 * lev] + T.laddr[2 * lev + 1]; b++) {
@@ -63,7 +65,8 @@ GemmLaunch tree_pass(const Tree &T, int lev, int nc, bool up)
     for (int o = 0; o < 8; o++)
         if (!s[o].empty()) {
             std::vector<double> one(s[o].size(), 1.0);
-            add_pairs(g, (int64_t)o * nc * nc, nc, s[o].data(), d[o].data(), one.data(), (int)s[o].size());
+            add_pairs(g, (int64_t)o * 2 * h * h, h, oct_phi[o], s[o].data(), d[o].data(), one.data(),
+                      (int)s[o].size());
         }
     return g;
 }
@@ -105,11 +108,11 @@ void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool 
         // M2M children at lev+1 -> parents at lev, for lev = nlev-1 .. 2;
         // L2L parents at lev -> children, for lev = 2 .. nlev-1
         for (int lev = T.nlev - 1; lev >= 2; lev--) {
-            GemmLaunch g = tree_pass(T, lev, nc, true);
+            GemmLaunch g = tree_pass(T, lev, nc, ops->oct_phi, true);
             if (!g.tiles.empty()) P.m2m_l.push_back(std::move(g));
         }
         for (int lev = 2; lev <= T.nlev - 1; lev++) {
-            GemmLaunch g = tree_pass(T, lev, nc, false);
+            GemmLaunch g = tree_pass(T, lev, nc, ops->oct_phi, false);
             if (!g.tiles.empty()) P.l2l_l.push_back(std::move(g));
         }
 

@@ -11,7 +11,7 @@ overlap); value = (triangles + targets) x K / wall time, latency = per
 request from submission to result. The same K applies back to back on one
 stream are reported as single_request. Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workers 3] [--iprec 2] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workers 2] [--iprec 2] [--no-cpu]
 """
 import argparse
 import json
@@ -85,7 +85,7 @@ def main():
     ap.add_argument("--ntarg", type=int, default=100000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", type=int, default=2000, help="targets checked against the direct sum")
-    ap.add_argument("--workers", type=int, default=3,
+    ap.add_argument("--workers", type=int, default=2,
                     help="requests in flight in the served measurement (1: single-request path only)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="run one device copy of MIB MiB first (HBM counter calibration under rocprofv3)")

@@ -30,6 +30,58 @@ void add_pairs(GemmLaunch &g, int64_t a_off, int h, double phi, const int *src, 
             g.tiles.push_back({a_off, h, r0, base + p0, std::min(ML_P, np - p0), phi});
 }
 
+// Launch order of translation tiles: workgroup b of a launch runs on XCD
+// b % 8 and takes tiles[b]. Tiles of one operator are cut into runs, the
+// runs go heaviest first to the least loaded XCD (work of a tile: its K
+// extent h plus the output adds, worth about 40 coefficients), and each
+// XCD takes its runs highest order first, so every XCD ends at about the
+// same time and an operator's tiles stay together in one XCD's L2.
+// Slots past an XCD's last tile hold empty tiles (npair = 0).
+void xcd_order(std::vector<M2LTile> &tiles)
+{
+    constexpr int NX = 8;
+    if (tiles.size() < 2 * NX) return;
+    auto work = [](const M2LTile &t) { return t.h + 40.0; };
+    double total = 0;
+    for (auto &t : tiles) total += work(t);
+    double cap = total / (NX * 16);
+    struct Run {
+        size_t b, e;
+        double w;
+        int h;
+    };
+    std::vector<Run> runs;
+    for (size_t i = 0; i < tiles.size(); i++) {
+        if (runs.empty() || tiles[i].a_off != tiles[runs.back().b].a_off || runs.back().w >= cap)
+            runs.push_back({i, i, 0.0, tiles[i].h});
+        runs.back().e = i + 1;
+        runs.back().w += work(tiles[i]);
+    }
+    std::vector<int> by_w(runs.size());
+    for (size_t r = 0; r < runs.size(); r++) by_w[r] = (int)r;
+    std::stable_sort(by_w.begin(), by_w.end(), [&](int a, int b) { return runs[a].w > runs[b].w; });
+    std::vector<int> bin[NX];
+    double load[NX] = {0};
+    for (int r : by_w) {
+        int x = (int)(std::min_element(load, load + NX) - load);
+        load[x] += runs[r].w;
+        bin[x].push_back(r);
+    }
+    size_t nmax = 0;
+    std::vector<M2LTile> seq[NX];
+    for (int x = 0; x < NX; x++) {
+        std::stable_sort(bin[x].begin(), bin[x].end(), [&](int a, int b) {
+            return runs[a].h != runs[b].h ? runs[a].h > runs[b].h : a < b;
+        });
+        for (int r : bin[x]) seq[x].insert(seq[x].end(), tiles.begin() + runs[r].b, tiles.begin() + runs[r].e);
+        nmax = std::max(nmax, seq[x].size());
+    }
+    M2LTile empty{0, 0, 0, 0, 0, 0.0};
+    tiles.assign(nmax * NX, empty);
+    for (int x = 0; x < NX; x++)
+        for (size_t i = 0; i < seq[x].size(); i++) tiles[i * NX + x] = seq[x][i];
+}
+
 // octant of child c in parent b: bit d set when c is the upper half in d
 inline int octant(const Tree &T, int c, int b)
 {
@@ -68,6 +120,7 @@ GemmLaunch tree_pass(const Tree &T, int lev, int nc, const double *oct_phi, bool
             add_pairs(g, (int64_t)o * 2 * h * h, h, oct_phi[o], s[o].data(), d[o].data(), one.data(),
                       (int)s[o].size());
         }
+    xcd_order(g.tiles);
     return g;
 }
 
@@ -193,6 +246,7 @@ void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool 
                 for (int r0 = 0; r0 < h; r0 += ML_M)
                     *t++ = M2LTile{ops->m2l_off[o], h, r0, ostart[o] + p0, std::min(ML_P, m - p0), ops->m2l_phi[o]};
         });
+        xcd_order(P.m2l_tiles);
         P.m2l.src.swap(ps);
         P.m2l.dst.swap(pd);
         P.m2l.scale.swap(psc);
@@ -358,7 +412,7 @@ void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool 
         npair_near += npts(P.nf_boxes[q]) * ns;
     }
     P.stats[6] = npair_near;
-    P.stats[7] = (double)P.m2l_tiles.size();
+    P.stats[7] = (double)std::count_if(P.m2l_tiles.begin(), P.m2l_tiles.end(), [](const M2LTile &t) { return t.npair > 0; });
     double m2m_pairs = 0, l2l_pairs = 0;
     for (auto &g : P.m2m_l) m2m_pairs += g.src.size();
     for (auto &g : P.l2l_l) l2l_pairs += g.src.size();

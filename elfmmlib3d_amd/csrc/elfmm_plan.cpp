@@ -158,6 +158,9 @@ void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool 
     if (ops) {
         for (int b = 0; b < nb; b++)
             if (!T.nkids[b] && T.level[b] >= 2 && T.src_count[b] > 0) P.p2m_boxes.push_back(b);
+        // one workgroup per box, most sources first
+        std::stable_sort(P.p2m_boxes.begin(), P.p2m_boxes.end(),
+                         [&](int a, int c) { return T.src_count[a] > T.src_count[c]; });
         // M2M children at lev+1 -> parents at lev, for lev = nlev-1 .. 2;
         // L2L parents at lev -> children, for lev = 2 .. nlev-1
         for (int lev = T.nlev - 1; lev >= 2; lev--) {
@@ -335,6 +338,31 @@ void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool 
         if (P.nf_items.size() == before) continue;
         P.nf_boxes.push_back(b);
         P.nf_start.push_back((int)P.nf_items.size());
+    }
+    {
+        // one workgroup per box, heaviest (points x triangles) first, so
+        // the long boxes do not start last
+        size_t nq = P.nf_boxes.size();
+        std::vector<double> w(nq);
+        std::vector<int> ord(nq);
+        for (size_t q = 0; q < nq; q++) {
+            double ns = 0;
+            for (int k = P.nf_start[q]; k < P.nf_start[q + 1]; k++) ns += T.src_count[P.nf_items[k]];
+            w[q] = ns * npts(P.nf_boxes[q]);
+            ord[q] = (int)q;
+        }
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return w[a] > w[b]; });
+        std::vector<int> bx(nq), st(1, 0), it;
+        it.reserve(P.nf_items.size());
+        for (size_t q = 0; q < nq; q++) {
+            int o = ord[q];
+            bx[q] = P.nf_boxes[o];
+            it.insert(it.end(), P.nf_items.begin() + P.nf_start[o], P.nf_items.begin() + P.nf_start[o + 1]);
+            st.push_back((int)it.size());
+        }
+        P.nf_boxes.swap(bx);
+        P.nf_start.swap(st);
+        P.nf_items.swap(it);
     }
     if (direct_only) P.nf_boxes.clear();
 

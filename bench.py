@@ -11,13 +11,14 @@ overlap); value = (triangles + targets) x K / wall time, latency = per
 request from submission to result. The same K applies back to back on one
 stream are reported as single_request. Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workers 2] [--iprec 2] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workers 3] [--iprec 2] [--no-cpu]
 """
 import argparse
 import json
 import os
 import sys
 import time
+from concurrent import futures
 
 import numpy as np
 
@@ -85,7 +86,7 @@ def main():
     ap.add_argument("--ntarg", type=int, default=100000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", type=int, default=2000, help="targets checked against the direct sum")
-    ap.add_argument("--workers", type=int, default=2,
+    ap.add_argument("--workers", type=int, default=3,
                     help="requests in flight in the served measurement (1: single-request path only)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="run one device copy of MIB MiB first (HBM counter calibration under rocprofv3)")
@@ -151,17 +152,20 @@ def main():
         for f in [srv.submit(req) for _ in range(max(args.warmup, args.workers))]:
             assert f.result().ier == 0
         torch.cuda.synchronize()
+        # closed loop: `workers` requests in flight, a slot refilled as soon
+        # as its request completes (in whatever order they complete)
         t0 = time.perf_counter()
-        pending, results, submitted = [], [], 0
+        pending, results, submitted = set(), [], 0
         while submitted < min(args.workers, args.steps):
-            pending.append(srv.submit(req))
+            pending.add(srv.submit(req))
             submitted += 1
         while pending:
-            r = pending.pop(0).result()
-            results.append(r)
-            if submitted < args.steps:
-                pending.append(srv.submit(req))
-                submitted += 1
+            done, pending = futures.wait(pending, return_when=futures.FIRST_COMPLETED)
+            for f in done:
+                results.append(f.result())
+                if submitted < args.steps:
+                    pending.add(srv.submit(req))
+                    submitted += 1
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         srv.close()
@@ -170,7 +174,8 @@ def main():
         diff = max(float(np.linalg.norm(a - b.cpu().numpy()) / np.linalg.norm(b.cpu().numpy()))
                    for a, b in zip(last, (out["ptfrc"], out["strain"], out["ptfrctarg"], out["straintarg"])))
         served = {"total_s": t1 - t0, "lat_ms": np.array([r.latency_ms for r in results]),
-                  "vs_single_rel": diff}
+                  "vs_single_rel": diff,
+                  "plan_ms": np.array([r.timings["plan"] for r in results])}
 
     if served:
         total = served["total_s"]
@@ -239,7 +244,9 @@ def main():
                    "tolerance": {1: 0.5e-3, 2: 0.5e-6, 3: 0.5e-9}.get(args.iprec)},
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
         "served": ({"workers": args.workers, "requests": args.steps,
-                    "result_vs_single_request_rel": served["vs_single_rel"]} if served else None),
+                    "result_vs_single_request_rel": served["vs_single_rel"],
+                    "host_plan_ms": {"p50": float(np.percentile(served["plan_ms"], 50)),
+                                     "max": float(served["plan_ms"].max())}} if served else None),
         "single_request": {"value": single_value, "ms_per_step": 1e3 * single_total / args.steps,
                            "latency_ms": {"p50": float(np.percentile(single_lat, 50)),
                                           "p99": float(np.percentile(single_lat, 99))}},

@@ -2,10 +2,14 @@
 // use, one parallel region at a time. std::thread rather than OpenMP so the
 // library brings no second OpenMP runtime into processes (PyTorch, the
 // Fortran runtime) that already carry one.
+#include <sched.h>
+
 #include <algorithm>
 #include <atomic>
 #include <condition_variable>
+#include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <functional>
 #include <mutex>
 #include <thread>
@@ -86,10 +90,33 @@ class Pool {
     std::atomic<int> next_{0};
 };
 
+// CPUs this process may use: its affinity mask, capped by a cgroup v2 CPU
+// quota (cpu.max "quota period") when one is set.
+int cpus_available()
+{
+    int n = (int)std::thread::hardware_concurrency();
+    cpu_set_t set;
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) n = CPU_COUNT(&set);
+    if (FILE *f = std::fopen("/sys/fs/cgroup/cpu.max", "r")) {
+        char q[32] = {0};
+        long period = 0;
+        if (std::fscanf(f, "%31s %ld", q, &period) == 2 && period > 0 && std::strcmp(q, "max") != 0) {
+            long quota = std::atol(q);
+            if (quota > 0) n = std::min(n, (int)((quota + period - 1) / period));
+        }
+        std::fclose(f);
+    }
+    return std::max(1, n);
+}
+
+// Default size: half the available CPUs, at most 8. The rest stay free for
+// the GPU runtime's threads and for the requests in flight beside the one
+// being planned; with every CPU of a quota in the pool, served throughput
+// fell by a quarter (CFS throttling of the whole process).
 Pool &pool()
 {
     static Pool *p = [] {
-        int n = (int)std::thread::hardware_concurrency();
+        int n = std::min(8, std::max(1, cpus_available() / 2));
         if (const char *e = std::getenv("ELFMM_HOST_THREADS")) n = std::atoi(e);
         n = std::max(1, std::min(n, 16));
         return new Pool(n);

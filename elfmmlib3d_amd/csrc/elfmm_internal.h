@@ -79,7 +79,7 @@ void build_operators(int p, const int *itable, Operators &ops);
 // the weight of the real field sum (1 for m = 0, else 2) and
 // 1/sqrt((l+m)!(l-m)!): the sum of c[t] * E[idx[t]], t < 3 (E as doubles,
 // re/im interleaved). Every such entry has at most three terms.
-struct FieldTerm {
+struct alignas(16) FieldTerm {   // 48 bytes; the device reads idx and c[0..1] as 16-byte vectors
     int idx[4];
     double c[4];
 };

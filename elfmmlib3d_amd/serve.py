@@ -4,10 +4,10 @@ Requests (one elfmm3dtriatarg or elfmm3dtriatargiter apply each, of any
 size) are admitted while a worker slot and device-memory budget are free;
 each worker is a host thread with its own HIP stream and its own library
 workspace, so one request's host planning (octree, lists, task lists)
-overlaps other requests' kernels. The library runs the kernels of
-different applies in the order the applies finish planning (each apply's
-first kernel waits for the last kernel of the one before), so a request
-that is planned runs alone on the GPU while the next one is planned. A request's result does not depend on what runs beside it (the
+overlaps other requests' kernels. The library runs the far fields of
+different applies in the order the applies finish planning (an apply's
+first far-field kernel waits for the last far-field kernel of the one
+before); near fields run on their own streams beside them. A request's result does not depend on what runs beside it (the
 library keeps no shared mutable state between applies except the cached
 translation operators and the iter stores a request names itself).
 

@@ -1,5 +1,5 @@
 """FMM apply throughput on one MI355X: elastostatic single+double layer on
-flat triangles (elfmm3dtriatarg), workload C2 of the baseline:
+flat triangles (elfmm3dtriatarg), workload C2:
   icosphere 64x64 per face (81,920 triangles), densities uniform in [-1,1]^3,
   evaluation at all centroids plus 100,000 targets uniform in [-2,2]^3 minus
   the unit ball, lam = mu = 1, FP64, iprec 2 (tolerance 0.5e-6).

@@ -1,7 +1,8 @@
 /*
  * CPU restatement of the reference's free-space triangle layer-potential
- * path. TEST INFRASTRUCTURE ONLY (the checker): tests/, the smoke check and
- * bench.py's cpu_baseline leg load it; the product never links it.
+ * path. TEST INFRASTRUCTURE ONLY: tests/, the smoke check and bench.py's
+ * cpu_baseline leg load it as the reference to compare against; the product
+ * never links it.
  *
  * Contents
  *  1. Plane-triangle potential tables (src/triahquad.f:680-1014). The

@@ -1,7 +1,7 @@
 /*
  * CPU restatement of the reference's free-space triangle layer-potential
  * path. TEST INFRASTRUCTURE ONLY: tests/, the smoke check and bench.py's
- * cpu_baseline leg load this as the checker; the product never links it.
+ * cpu_baseline leg load this to compare against; the product never links it.
  *
  * Arrays use the Fortran layouts of the reference: triangle(3,3,n) is
  * tri[9*k + 3*v + c], vectors (3,n) are x[3*k + c], strain(3,3,n) is

@@ -1,7 +1,7 @@
 """ctypes driver for the C restatement (oracle/_build/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: tests/, the smoke check and bench.py's
-cpu_baseline leg use it as the checker; the product never imports it.
+cpu_baseline leg use it to compare against; the product never imports it.
 Build with ``make -C oracle oracle``.
 """
 from __future__ import annotations

@@ -1,7 +1,7 @@
 """ctypes driver for the reference library compiled from its Fortran sources.
 
-TEST INFRASTRUCTURE ONLY. This module is the checker: tests/, the smoke check
-and bench.py's cpu_baseline leg may load it; the product path never does.
+TEST INFRASTRUCTURE ONLY: tests/, the smoke check and bench.py's cpu_baseline
+leg may load it to compare against; the product path never does.
 
 ``oracle/_ref/libelfmm_ref.so`` is built by ``make -C oracle ref`` from the
 reference's sources where they lie (see oracle/Makefile). Every Fortran

@@ -92,7 +92,7 @@ def main():
                     help="run one device copy of MIB MiB first (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "the library runs on one GPU per process; --gpus must be 1"}))
         sys.exit(2)
 
     import torch

@@ -70,15 +70,16 @@ class Result:
 
 
 def estimate_bytes(n, nt, p=26):
-    """Device memory of one apply's workspace (expansions, nodes, fields,
-    records, inputs/outputs), a conservative estimate used for admission."""
+    """Device memory of one apply's workspace (expansions, quadrature nodes,
+    near-field records, per-point fields, inputs/outputs), a conservative
+    estimate used for admission. The L2T/M2T field operands are formed in
+    LDS and take no device memory."""
     nboxes = 2 * (n + nt) // 40 + 64
     nc = (p + 1) * (p + 2)
     exps = 2 * nboxes * 4 * nc * 8
-    fields = nboxes * 4 * 9 * (p + 3) * (p + 4) * 8
     pointwise = (n + nt) * (40 + 9 + 3 + 24) * 8 + n * (12 * 20 + 18 + 24) * 8
     io = (n * 27 + nt * 15) * 8
-    return int(exps + fields + pointwise + io) + (64 << 20)
+    return int(exps + pointwise + io) + (64 << 20)
 
 
 class Server:

@@ -34,7 +34,7 @@ NEAR_OPS_PER_PAIR = 862 + 1441 + 31 + 25
 # HBM bytes per launch on this workload, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE, each scaled by what the counter reads for a 512 MiB device copy
 # in the same run (profiles/c2_apply_kernels.md)
-TRAFFIC = {"k_near": 211.5e6, "k_m2l": 8306.0e6}
+TRAFFIC = {"k_near": 211.5e6, "k_m2l": 8148.0e6}
 
 
 def workload(nsplit=64, ntarg=100000, seed=2024):

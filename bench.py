@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--check", type=int, default=2000, help="targets checked against the direct sum")
     ap.add_argument("--workers", type=int, default=3,
                     help="requests in flight in the served measurement (1: single-request path only)")
+    ap.add_argument("--no-iter-replay", dest="iter_replay", action="store_false",
+                    help="skip the stored near-field replay measurement")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="run one device copy of MIB MiB first (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -210,6 +212,42 @@ def main():
                "strain_rel_l2": float(np.linalg.norm(fs - dst) / np.linalg.norm(dst)),
                "checked_targets": int(len(idx))}
 
+    # iter engine (elfmm3dtriatargiter): the near field stored once (icomp 2)
+    # and replayed with the densities (icomp 3), the matvec of an iterative
+    # solve. The replay streams the store from HBM once per apply: bytes per
+    # launch = stored words x 8 over the replay kernel's time (near field on
+    # the apply's own stream). Workload: the mesh above at its centroids,
+    # SL + DL displacement. Outside the timed region.
+    replay = None
+    if args.iter_replay:
+        os.environ["ELFMM_SERIAL_NEAR"] = "1"
+        ws = None
+        try:
+            kw = dict(ifptfrc=1, ifstrain=0)
+            ier, lused, *_ = E.elfmm3dtriatargiter(args.iprec, 1.0, 1.0, tri, nrm, src, ssl, sdl, None,
+                                                   icomp_type=4, **kw)
+            ws = np.zeros(max(lused, 4))
+            ier, *_ = E.elfmm3dtriatargiter(args.iprec, 1.0, 1.0, tri, nrm, src, ssl, sdl, None,
+                                            icomp_type=2, wsave=ws, **kw)
+            words = E.last_stats()["iter_store_words"]
+            tms = []
+            for _ in range(3):
+                ier, *_ = E.elfmm3dtriatargiter(args.iprec, 1.0, 1.0, tri, nrm, src, ssl, sdl, None,
+                                                icomp_type=3, wsave=ws, **kw)
+                if ier != 0:
+                    raise RuntimeError(f"iter replay ier={ier}")
+                tms.append(E.last_timings()["near"])
+            ms = float(np.median(tms))
+            bts = words * 8.0
+            replay = {"kernel": "k_near_replay", "bound": "hbm", "achieved": bts / ms / 1e6,
+                      "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": bts / (ms / 1e3) / PEAK_HBM,
+                      "traffic": None, "algorithmic_bytes_per_launch": bts, "ms_per_launch": ms,
+                      "workload": f"{n} triangles at their centroids, SL+DL displacement, iprec {args.iprec}"}
+        finally:
+            if ws is not None:
+                E.iter_release(ws)
+            del os.environ["ELFMM_SERIAL_NEAR"]
+
     # dominant kernel and its roofline. Algorithmic work per launch:
     #  near field: near (point, triangle) pairs x NEAR_OPS_PER_PAIR, the
     #    counted operations of one pair of the library's pair quadrature
@@ -252,7 +290,8 @@ def main():
                                           "p99": float(np.percentile(single_lat, 99))}},
         "accuracy_vs_direct": err,
         "phases_ms": ph, "kernel_phases_ms": kph, "work": stats,
-        "roofline": roof, "rooflines": rooflines, "cpu_baseline": cpu,
+        "roofline": roof, "rooflines": dict(rooflines, **({"iter_replay": replay} if replay else {})),
+        "cpu_baseline": cpu,
     }
     if cpu:
         line["speedup_vs_cpu_reference"] = value / cpu["value"]

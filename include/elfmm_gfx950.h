@@ -105,8 +105,11 @@ int elfmm_gfx950_iter_release(const double *wsave, int lwsave);
 /* Device-resident elfmm3dtriatarg: all arrays are device pointers with the
  * Fortran layouts above; the call is enqueued on `stream` (a hipStream_t,
  * NULL for the default stream) and returns after the outputs are written.
- * No counterpart in the reference: it is the same operation for callers
- * whose data already lives in HBM. */
+ * Calls from several host threads, each on its own stream, may run at
+ * once: their host planning takes turns, and their far fields run on the
+ * device in the order the calls finish planning, while near fields run
+ * beside them. No counterpart in the reference: it is the same operation
+ * for callers whose data already lives in HBM. */
 int elfmm_gfx950_triatarg_dev(int iprec, double rlam, double rmu, int nparts,
                               const double *d_triangle, const double *d_trinorm,
                               const double *d_source, int ifsingle, const double *d_sigma_sl,

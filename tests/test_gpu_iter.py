@@ -124,3 +124,35 @@ def test_iter_on_surface_entry(gpu_lib):
     ref = gpu_lib.elfmm3dtriatargiter(int(g["iprec"]), float(g["lam"]), float(g["mu"]), tri, nrm, src, ssl,
                                       sdl, None, icomp_type=1)
     assert rel(pt, ref[2]) < 1e-13 and rel(st, ref[3]) < 1e-13
+
+
+@pytest.mark.parametrize("layers", ["sl", "dl", "both"])
+@pytest.mark.parametrize("fields", ["disp", "strain", "both"])
+def test_iter_replay_every_store_layout(gpu_lib, layers, fields):
+    """Replay (icomp 3) with new densities equals a direct compute (icomp 1,
+    pinned to the reference above) for each stored layout: one or both
+    layers, displacement and/or strain fields."""
+    g = load_golden("iter_c1_p1")
+    ssl2 = g["sigma_sl2"] if layers in ("sl", "both") else None
+    sdl2 = g["sigma_dl2"] if layers in ("dl", "both") else None
+    ssl1 = g["sigma_sl"] if ssl2 is not None else None
+    sdl1 = g["sigma_dl"] if sdl2 is not None else None
+    fd, fs = int(fields != "strain"), int(fields != "disp")
+    flags = dict(ifptfrc=fd, ifstrain=fs, ifptfrctarg=fd, ifstraintarg=fs)
+
+    def call(icomp, ssl, sdl, wsave=None):
+        return gpu_lib.elfmm3dtriatargiter(int(g["iprec"]), float(g["lam"]), float(g["mu"]), g["triangle"],
+                                           g["trinorm"], g["source"], ssl, sdl, g["target"], icomp_type=icomp,
+                                           wsave=wsave, **flags)
+
+    print(f"layers {layers} fields {fields}", flush=True)
+    ier, lused, *_ = call(4, ssl1, sdl1)
+    wsave = np.zeros(max(lused, 4))
+    assert call(2, ssl1, sdl1, wsave)[0] == 0
+    out3 = call(3, ssl2, sdl2, wsave)
+    out1 = call(1, ssl2, sdl2)
+    assert out3[0] == 0 and out1[0] == 0
+    for a, b, on in zip(out3[2:], out1[2:], (fd, fs, fd, fs)):
+        if on:
+            assert rel(a, b) < 1e-12
+    assert gpu_lib.iter_release(wsave) == 1

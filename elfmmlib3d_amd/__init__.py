@@ -1,7 +1,8 @@
 """MI355X (gfx950) elastostatic triangle layer-potential FMM.
 
 Python mirror of the reference's user-callable subroutines
-(src/elfmm3dtria.f:71, :162, :1453; src/elfmm3dtria_iter_1.f:160), calling the C-ABI of
+(src/elfmm3dtria.f:71, :162, :1453; src/elfmm3dtria_iter_1.f:160;
+src/elhfmm3dtria_iter.f:46; src/elh3dtria.f:53), calling the C-ABI of
 ``libelfmm_gfx950.so`` (declared in include/elfmm_gfx950.h). Arrays use the
 shapes of mesh.py: triangle (n,3,3), trinorm/source/sigma (n,3), target (m,3);
 outputs ptfrc (n,3), strain (n,3,3). The library is the only compute path:
@@ -11,6 +12,9 @@ from __future__ import annotations
 
 from ._lib import (  # noqa: F401
     el3dtriadirecttarg,
+    elh3dtriadirecttarg,
+    elhfmm3dtriatargiter,
+    half_parts,
     elfmm3dtria,
     elfmm3dtriatarg,
     elfmm3dtriatarg_dev,
@@ -24,5 +28,6 @@ from ._lib import (  # noqa: F401
 from ._build import build  # noqa: F401
 
 __all__ = ["elfmm3dtria", "elfmm3dtriatarg", "el3dtriadirecttarg", "elfmm3dtriatarg_dev",
-           "elfmm3dtriatargiter", "iter_release",
+           "elfmm3dtriatargiter", "elhfmm3dtriatargiter", "elh3dtriadirecttarg", "half_parts",
+           "iter_release",
            "last_stats", "last_timings", "library", "library_path", "build"]

@@ -33,6 +33,7 @@ def library():
         _lib = C.CDLL(LIB)
         _lib.elfmm_gfx950_triatarg_dev.restype = C.c_int
         _lib.elfmm_gfx950_triatargiter_dev.restype = C.c_int
+        _lib.elfmm_gfx950_htriatargiter_dev.restype = C.c_int
         _lib.elfmm_gfx950_version.restype = C.c_char_p
     return _lib
 
@@ -146,6 +147,50 @@ def elfmm3dtriatargiter(iprec, rlam, rmu, triangle, trinorm, source, sigma_sl=No
                                    _iref(ifstraintarg if m else 0), _p(stt), _iref(icomp_type),
                                    _p(wsave), _iref(wsave.size), C.byref(lused))
     return ier.value, lused.value, pt, st, ptt[:m], stt[:m]
+
+
+def elhfmm3dtriatargiter(iprec, rlam, rmu, triangle, trinorm, source, sigma_sl=None, sigma_dl=None,
+                         target=None, ifptfrctarg=1, ifstraintarg=1, icomp_type=1, wsave=None):
+    """Half-space (Mindlin) FMM, image part, at the targets
+    (src/elhfmm3dtria_iter.f:46): icomp_type as elfmm3dtriatargiter (the
+    image near field is what 2 keeps and 3 replays). Returns
+    (ier, lused, ptfrctarg, straintarg)."""
+    n, tri, nrm, src, ifs, ssl, ifd, sdl, m, tgt = _prep(triangle, trinorm, source, sigma_sl,
+                                                          sigma_dl, target)
+    ptt = np.zeros((max(m, 1), 3))
+    stt = np.zeros((max(m, 1), 3, 3))
+    if wsave is None:
+        wsave = np.zeros(1)
+    ier, lused = C.c_int(0), C.c_int(0)
+    library().elhfmm3dtriatargiter_(C.byref(ier), _iref(iprec), _dref(rlam), _dref(rmu), _p(tri), _p(nrm),
+                                    _iref(n), _p(src), _iref(ifs), _p(ssl), _iref(ifd), _p(sdl), _iref(m),
+                                    _p(tgt), _iref(ifptfrctarg if m else 0), _p(ptt),
+                                    _iref(ifstraintarg if m else 0), _p(stt), _iref(icomp_type),
+                                    _p(wsave), _iref(wsave.size), C.byref(lused))
+    return ier.value, lused.value, ptt[:m], stt[:m]
+
+
+def elh3dtriadirecttarg(rlam, rmu, triangle, trinorm, source, sigma_sl=None, sigma_dl=None,
+                        target=None, ifptfrc=0, ifstrain=0, ifptfrctarg=1, ifstraintarg=1):
+    """O(N^2) half-space direct sum on the GPU (src/elh3dtria.f:53): free space
+    plus the adaptive image integral of every pair.
+    Returns (ptfrc, strain, ptfrctarg, straintarg)."""
+    n, tri, nrm, src, ifs, ssl, ifd, sdl, m, tgt = _prep(triangle, trinorm, source, sigma_sl,
+                                                          sigma_dl, target)
+    pt, st = np.zeros((n, 3)), np.zeros((n, 3, 3))
+    ptt, stt = np.zeros((max(m, 1), 3)), np.zeros((max(m, 1), 3, 3))
+    library().elh3dtriadirecttarg_(_dref(rlam), _dref(rmu), _p(tri), _p(nrm), _iref(n), _p(src),
+                                   _iref(ifs), _p(ssl), _iref(ifd), _p(sdl), _iref(ifptfrc), _p(pt),
+                                   _iref(ifstrain), _p(st), _iref(m), _p(tgt),
+                                   _iref(ifptfrctarg if m else 0), _p(ptt),
+                                   _iref(ifstraintarg if m else 0), _p(stt))
+    return pt, st, ptt[:m], stt[:m]
+
+
+def half_parts(mask=7):
+    """Diagnostic: which parts of the half-space apply run (bit 0 image near
+    field, bit 1 anti-image far field, bit 2 Mindlin far field)."""
+    library().elfmm_gfx950_half_parts(C.c_int(mask))
 
 
 def iter_release(wsave):

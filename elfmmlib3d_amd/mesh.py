@@ -142,17 +142,18 @@ def cube_targets(m, half, rexclude, rng):
     return np.ascontiguousarray(out[:m])
 
 
-def fault_mesh(nfaults, ntri_total, rng, box=20.0):
+def fault_mesh(nfaults, ntri_total, rng, box=20.0, top=(0.0, 3.0)):
     """Planar rectangular faults in z<0 meshed into right triangles.
 
-    Length U[2,10], width U[1,5], dip U[15,90] deg, top depth U[0,3], random
-    strike and centre in a box x box square; a uniform cell size h is chosen
-    so that the total triangle count is close to ntri_total.
+    Length U[2,10], width U[1,5], dip U[15,90] deg, top depth U[top] (default
+    U[0,3]; every fault starts 1e-3 below it), random strike and centre in a
+    box x box square; a uniform cell size h is chosen so that the total
+    triangle count is close to ntri_total.
     """
     L = rng.uniform(2.0, 10.0, nfaults)
     W = rng.uniform(1.0, 5.0, nfaults)
     dip = np.deg2rad(rng.uniform(15.0, 90.0, nfaults))
-    top = rng.uniform(0.0, 3.0, nfaults)
+    top = rng.uniform(top[0], top[1], nfaults)
     strike = rng.uniform(0.0, 2.0 * np.pi, nfaults)
     cx = rng.uniform(-box / 2, box / 2, nfaults)
     cy = rng.uniform(-box / 2, box / 2, nfaults)

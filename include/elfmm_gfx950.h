@@ -102,6 +102,56 @@ int elfmm_gfx950_triatargiter_dev(int iprec, double rlam, double rmu, int nparts
  * memory. */
 int elfmm_gfx950_iter_release(const double *wsave, int lwsave);
 
+/* Replaces elhfmm3dtriatargiter (src/elhfmm3dtria_iter.f:46-51): the
+ * elastostatic layer potentials of the Mindlin half space z < 0, image part
+ * only (the reference's caller adds the free-space "direct arrival" from
+ * elfmm3dtriatargiter with the same tree rule; sources are evaluated by
+ * passing them as targets). Result = anti-image far field (subtracted) +
+ * Mindlin harmonic far field + image near field of lists 1, 3, 4 and self,
+ * integrated adaptively (1e-6 relative, depth <= 200) on the GPU.
+ * icomp_type as elfmm3dtriatargiter_ (1 compute, 2 compute and keep the
+ * image near field on the device, 3 replay it, 4 plan only, 5 keep it and
+ * return the near field only, 0 zero outputs); wsave (lwsave >= 4) carries
+ * the handle, released by elfmm_gfx950_iter_release. Deviation: the
+ * reference's integrand leaves the output entries it was not asked for
+ * unset (src/elhfmm3dtria_iter.f:2901-3027) and its subdivision test reads
+ * them; here they are zero, so with only one of ifptfrctarg/ifstraintarg
+ * set the subdivision can differ from a given run of the reference. */
+void elhfmm3dtriatargiter_(int *ier, const int *iprec, const double *rlam, const double *rmu,
+                           const double *triangle, const double *trinorm, const int *nparts,
+                           const double *source, const int *ifsingle, const double *sigma_sl,
+                           const int *ifdouble, const double *sigma_dl, const int *ntargs,
+                           const double *target, const int *ifptfrctarg, double *ptfrctarg,
+                           const int *ifstraintarg, double *straintarg, const int *icomp_type,
+                           double *wsave, const int *lwsave, int *lused);
+
+/* Device-resident elhfmm3dtriatargiter (device arrays, host wsave, a HIP
+ * stream); returns ier. No counterpart in the reference. */
+int elfmm_gfx950_htriatargiter_dev(int iprec, double rlam, double rmu, int nparts,
+                                   const double *d_triangle, const double *d_trinorm,
+                                   const double *d_source, int ifsingle, const double *d_sigma_sl,
+                                   int ifdouble, const double *d_sigma_dl, int ntargs,
+                                   const double *d_target, int ifptfrctarg, double *d_ptfrctarg,
+                                   int ifstraintarg, double *d_straintarg, int icomp_type,
+                                   double *wsave, int lwsave, int *lused, void *stream);
+
+/* Replaces elh3dtriadirecttarg (src/elh3dtria.f:53-58): the O(N^2) half-space
+ * direct sum (free-space pair quadratures plus the adaptive image integral
+ * of every pair), on the GPU. */
+void elh3dtriadirecttarg_(const double *rlam, const double *rmu, const double *triangle,
+                          const double *trinorm, const int *nsource, const double *source,
+                          const int *ifsingle, const double *sigma_sl, const int *ifdouble,
+                          const double *sigma_dl, const int *ifptfrc, double *ptfrc,
+                          const int *ifstrain, double *strain, const int *ntarget,
+                          const double *target, const int *ifptfrctarg, double *ptfrctarg,
+                          const int *ifstraintarg, double *straintarg);
+
+/* Diagnostic: which parts of the half-space apply run (bit 0 image near
+ * field, bit 1 anti-image far field, bit 2 Mindlin harmonic far field;
+ * default 7), process-wide. Lets a test compare each part with the
+ * reference's routine for that part. No counterpart in the reference. */
+void elfmm_gfx950_half_parts(int mask);
+
 /* Device-resident elfmm3dtriatarg: all arrays are device pointers with the
  * Fortran layouts above; the call is enqueued on `stream` (a hipStream_t,
  * NULL for the default stream) and returns after the outputs are written.

@@ -278,3 +278,135 @@ def elfmm3dtriatargiter(iprec, rlam, rmu, triangle, trinorm, source,
         _i(ifstraintarg if m else 0), _p(stt), _i(icomp_type), _p(wsave), _i(wsave.size),
         C.byref(lused))
     return ier.value, lused.value, pt, st, ptt[:m], stt[:m], wsave
+
+
+def _hs_inputs(triangle, trinorm, source, sigma_sl, sigma_dl, target):
+    n = triangle.shape[0]
+    tri = _f64(triangle, (n, 3, 3))
+    nrm = _f64(trinorm, (n, 3))
+    src = _f64(source, (n, 3))
+    ifs = 0 if sigma_sl is None else 1
+    ifd = 0 if sigma_dl is None else 1
+    ssl = _f64(sigma_sl if ifs else np.zeros((n, 3)), (n, 3))
+    sdl = _f64(sigma_dl if ifd else np.zeros((n, 3)), (n, 3))
+    m = target.shape[0]
+    tgt = _f64(target if m else np.zeros((1, 3)), (max(m, 1), 3))
+    return n, tri, nrm, src, ifs, ssl, ifd, sdl, m, tgt
+
+
+def elhfmm3dtriatargiter(iprec, rlam, rmu, triangle, trinorm, source, sigma_sl=None, sigma_dl=None,
+                         target=None, ifptfrctarg=1, ifstraintarg=1, icomp_type=1, wsave=None):
+    """Reference half-space FMM (src/elhfmm3dtria_iter.f:46). Returns
+    (ier, lused, ptfrctarg, straintarg, wsave)."""
+    n, tri, nrm, src, ifs, ssl, ifd, sdl, m, tgt = _hs_inputs(triangle, trinorm, source, sigma_sl, sigma_dl,
+                                                              target)
+    ptt = np.zeros((max(m, 1), 3))
+    stt = np.zeros((max(m, 1), 3, 3))
+    if wsave is None:
+        wsave = np.zeros(1)
+    ier = C.c_int(0)
+    lused = C.c_int(0)
+    libh().elhfmm3dtriatargiter_(
+        C.byref(ier), _i(iprec), _d(rlam), _d(rmu), _p(tri), _p(nrm), _i(n), _p(src), _i(ifs), _p(ssl),
+        _i(ifd), _p(sdl), _i(m), _p(tgt), _i(ifptfrctarg), _p(ptt), _i(ifstraintarg), _p(stt),
+        _i(icomp_type), _p(wsave), _i(wsave.size), C.byref(lused))
+    return ier.value, lused.value, ptt[:m], stt[:m], wsave
+
+
+def elhfmm3d_near(iprec, rlam, rmu, triangle, trinorm, source, sigma_sl=None, sigma_dl=None, target=None,
+                  ifptfrctarg=1, ifstraintarg=1, icomp_type=1):
+    """The half-space image near field alone, elhfmm3dtriatargiter0
+    (src/elhfmm3dtria_iter.f:1387), called as the entry point calls it
+    (targets mirrored, :252-258) with a work array large enough for every
+    icomp. Returns (ier, lused, ptfrctarg, straintarg)."""
+    n, tri, nrm, src, ifs, ssl, ifd, sdl, m, tgt = _hs_inputs(triangle, trinorm, source, sigma_sl, sigma_dl,
+                                                              target)
+    tgtm = tgt.copy()
+    tgtm[:, 2] *= -1
+    ptt = np.zeros((max(m, 1), 3))
+    stt = np.zeros((max(m, 1), 3, 3))
+    pt = np.zeros(3)
+    st = np.zeros(9)
+    ier = C.c_int(0)
+    lused = C.c_int(0)
+    zero = C.c_int(0)
+    args = lambda icomp, ws: (C.byref(ier), _i(iprec), _d(rlam), _d(rmu), _p(tri), _p(nrm), _i(n), _p(src),
+                              _i(ifs), _p(ssl), _i(ifd), _p(sdl), _p(nrm), C.byref(zero), _p(pt),
+                              C.byref(zero), _p(st), _i(m), _p(tgtm), _i(ifptfrctarg), _p(ptt),
+                              _i(ifstraintarg), _p(stt), _i(icomp), _p(ws), _i(ws.size), C.byref(lused))
+    ws = np.zeros(1)
+    libh().elhfmm3dtriatargiter0_(*args(4, ws))
+    need = lused.value
+    ws = np.zeros(max(need, 1) + 16)
+    libh().elhfmm3dtriatargiter0_(*args(icomp_type, ws))
+    return ier.value, need, ptt[:m].copy(), stt[:m].copy()
+
+
+def elxfmm3d_anti(iprec, rlam, rmu, triangle, trinorm, source, sigma_sl=None, sigma_dl=None, target=None,
+                  ifptfrctarg=1, ifstraintarg=1):
+    """The half-space anti-image far field alone, as elhfmm3dtriatarg_image
+    uses it (src/elhfmm3dtria_iter.f:747-777): elxfmm3dtriatarg with
+    (lam + 4 mu, -mu), -sigma_dl and the mirrored targets, subtracted.
+    Returns (ier, ptfrctarg, straintarg) with the subtraction applied."""
+    n, tri, nrm, src, ifs, ssl, ifd, sdl, m, tgt = _hs_inputs(triangle, trinorm, source, sigma_sl, sigma_dl,
+                                                              target)
+    tgtm = tgt.copy()
+    tgtm[:, 2] *= -1
+    sdlm = -sdl
+    ptt = np.zeros((max(m, 1), 3))
+    stt = np.zeros((max(m, 1), 3, 3))
+    pt = np.zeros(3)
+    st = np.zeros(9)
+    ier = C.c_int(0)
+    libh().elxfmm3dtriatarg_(
+        C.byref(ier), _i(iprec), _d(rlam + 4 * rmu), _d(-rmu), _p(tri), _p(nrm), _i(n), _p(src), _i(ifs),
+        _p(ssl), _i(ifd), _p(sdlm), _i(0), _p(pt), _i(0), _p(st), _i(m), _p(tgtm), _i(ifptfrctarg), _p(ptt),
+        _i(ifstraintarg), _p(stt))
+    return ier.value, -ptt[:m].copy(), -stt[:m].copy()
+
+
+def lfmm3d_mindlin(iprec, rlam, rmu, triangle, trinorm, source, sigma_sl=None, sigma_dl=None, target=None):
+    """The Mindlin harmonic-image FMM alone, as elhfmm3dtriatarg_image uses
+    it (src/elhfmm3dtria_iter.f:795-831): lfmm3dmindlintriatarg on the
+    mirrored triangles, its potential / mu added to the displacement and
+    -sym(field) / (2 mu) to the strain. Returns (ier, ptfrctarg, straintarg)."""
+    n, tri, nrm, src, ifs, ssl, ifd, sdl, m, tgt = _hs_inputs(triangle, trinorm, source, sigma_sl, sigma_dl,
+                                                              target)
+    trim = tri.copy()
+    trim[:, :, 2] *= -1
+    nrmm = nrm.copy()
+    nrmm[:, 2] *= -1
+    srcm = src.copy()
+    srcm[:, 2] *= -1
+    pot = np.zeros((max(m, 1), 3), dtype=np.complex128)
+    fld = np.zeros((max(m, 1), 3, 3), dtype=np.complex128)
+    rlame = np.array([rlam, rmu], dtype=np.float64)
+    ier = C.c_int(0)
+    libh().lfmm3dmindlintriatarg_(
+        C.byref(ier), _i(iprec), _i(n), _p(trim), _p(nrmm), _p(srcm), _p(rlame), _i(ifs), _p(ssl), _i(ifd),
+        _p(sdl), _p(nrm), _i(m), _p(tgt), _i(1), _p(pot), _i(1), _p(fld))
+    pt = pot[:m].real / rmu
+    f = fld[:m].real        # f[t, icomp, k] = Fortran fld(k, icomp, t)
+    st = np.zeros((m, 3, 3))
+    for ic in range(3):
+        for k in range(3):
+            # straintarg(k, ic) -= fld(k, ic)/(2mu); straintarg(ic, k) -= fld(k, ic)/(2mu)
+            st[:, ic, k] -= f[:, ic, k] / (2 * rmu)
+            st[:, k, ic] -= f[:, ic, k] / (2 * rmu)
+    return ier.value, pt, st
+
+
+def elh3dtriadirecttarg(rlam, rmu, triangle, trinorm, source, sigma_sl=None, sigma_dl=None, target=None,
+                        ifptfrc=0, ifstrain=0, ifptfrctarg=1, ifstraintarg=1):
+    """Reference half-space direct sum (src/elhfmm3dtria_iter.f:3033, the copy
+    of src/elh3dtria.f:53). Returns (ptfrc, strain, ptfrctarg, straintarg)."""
+    n, tri, nrm, src, ifs, ssl, ifd, sdl, m, tgt = _hs_inputs(triangle, trinorm, source, sigma_sl, sigma_dl,
+                                                              target)
+    pt = np.zeros((n, 3))
+    st = np.zeros((n, 3, 3))
+    ptt = np.zeros((max(m, 1), 3))
+    stt = np.zeros((max(m, 1), 3, 3))
+    libh().elh3dtriadirecttarg_(
+        _d(rlam), _d(rmu), _p(tri), _p(nrm), _i(n), _p(src), _i(ifs), _p(ssl), _i(ifd), _p(sdl), _i(ifptfrc),
+        _p(pt), _i(ifstrain), _p(st), _i(m), _p(tgt), _i(ifptfrctarg), _p(ptt), _i(ifstraintarg), _p(stt))
+    return pt, st, ptt[:m], stt[:m]

@@ -27,6 +27,17 @@ never need the reference itself. Cases (all FP64):
   tables        random plane-triangle tables (triabtable / triartable)
   tree_c1       boxes, permutations and lists of d3tstrcr on C1 points
   direct_s4     el3dtriadirecttarg on a 320-triangle sphere + 50 targets
+  hs_fault_p2   half space: 4 planar faults in z<0 (~2000 right triangles,
+                top depths from 0.05), targets = the centroids + 150 points
+                around the faults (z<0), lam=1, mu=0.8, iprec 2, single +
+                double layer: elhfmm3dtriatargiter icomp 1 (lwsave >= lused)
+                and icomp 4 (lused); each part alone: image near field
+                (elhfmm3dtriatargiter0, icomp 1 and 2, the second also for a
+                second density set), anti-image far field (elxfmm3dtriatarg,
+                subtracted), Mindlin far field (lfmm3dmindlintriatarg);
+                elh3dtriadirecttarg and the free-space iter FMM (direct
+                arrival) at 24 of the targets
+  hs_fault_p1   the same geometry at iprec 1
 """
 from __future__ import annotations
 
@@ -100,6 +111,56 @@ def iter_case(name, iprec, lam=1.2, mu=0.7):
          sigma_dl=sdl, sigma_sl2=ssl2, sigma_dl2=sdl2, target=tg, lused=lused, **out)
 
 
+def hs_geometry(seed=21, ntri=2000, ntarg=150):
+    rng = np.random.default_rng(seed)
+    tri = mesh.fault_mesh(4, ntri, rng, box=10.0, top=(0.05, 2.0))
+    nrm = mesh.triangle_normals(tri)
+    src = mesh.centroids(tri)
+    n = tri.shape[0]
+    ssl = rng.uniform(-1, 1, (n, 3))
+    sdl = rng.uniform(-1, 1, (n, 3))
+    ssl2 = rng.uniform(-1, 1, (n, 3))
+    sdl2 = rng.uniform(-1, 1, (n, 3))
+    pick = rng.integers(0, n, ntarg)
+    off = src[pick] + rng.normal(0, 0.4, (ntarg, 3))
+    off[:, 2] = -np.abs(off[:, 2]) - 0.01
+    tg = np.concatenate([src, off])
+    return tri, nrm, src, ssl, sdl, ssl2, sdl2, tg
+
+
+def hs_case(name, iprec, lam=1.0, mu=0.8):
+    tri, nrm, src, ssl, sdl, ssl2, sdl2, tg = hs_geometry()
+    out = {}
+    ier, lused, _, _, _ = ref.elhfmm3dtriatargiter(iprec, lam, mu, tri, nrm, src, ssl, sdl, tg, icomp_type=4)
+    assert ier == 0
+    out["lused"] = lused
+    ws = np.zeros(lused + 16)
+    ier, _, out["full_pt"], out["full_st"], _ = ref.elhfmm3dtriatargiter(iprec, lam, mu, tri, nrm, src, ssl, sdl,
+                                                                         tg, icomp_type=1, wsave=ws)
+    assert ier == 0
+    del ws
+    for ic in (1, 2):
+        ier, need, out[f"near{ic}_pt"], out[f"near{ic}_st"] = ref.elhfmm3d_near(iprec, lam, mu, tri, nrm, src, ssl,
+                                                                                 sdl, tg, icomp_type=ic)
+        assert ier == 0 and need == lused
+    _, _, out["near2b_pt"], out["near2b_st"] = ref.elhfmm3d_near(iprec, lam, mu, tri, nrm, src, ssl2, sdl2, tg,
+                                                                 icomp_type=2)
+    ier, out["anti_pt"], out["anti_st"] = ref.elxfmm3d_anti(iprec, lam, mu, tri, nrm, src, ssl, sdl, tg)
+    assert ier == 0
+    ier, out["mind_pt"], out["mind_st"] = ref.lfmm3d_mindlin(iprec, lam, mu, tri, nrm, src, ssl, sdl, tg)
+    assert ier == 0
+    sub = np.linspace(0, tg.shape[0] - 1, 24).astype(int)
+    _, _, out["direct_pt"], out["direct_st"] = ref.elh3dtriadirecttarg(lam, mu, tri, nrm, src, ssl, sdl, tg[sub])
+    r = ref.elfmm3dtriatargiter(iprec, lam, mu, tri, nrm, src, ssl, sdl, tg[sub], ifptfrc=0, ifstrain=0,
+                                icomp_type=4)
+    ws = np.zeros(r[1] + 16)
+    r = ref.elfmm3dtriatargiter(iprec, lam, mu, tri, nrm, src, ssl, sdl, tg[sub], ifptfrc=0, ifstrain=0,
+                                icomp_type=1, wsave=ws)
+    out["arrival_pt"], out["arrival_st"] = r[4], r[5]
+    save(name, iprec=iprec, lam=lam, mu=mu, triangle=tri, trinorm=nrm, source=src, sigma_sl=ssl, sigma_dl=sdl,
+         sigma_sl2=ssl2, sigma_dl2=sdl2, target=tg, direct_idx=sub, **out)
+
+
 def main(only=None):
     want = lambda name: only is None or name in only
     tri, nrm, src, ssl, sdl, tg = sphere_case(6, 300, 11)
@@ -121,6 +182,10 @@ def main(only=None):
     if want("fmm_tiny"):
         t2, n2, s2, sl2, dl2, tg2 = sphere_case(2, 10, 5)
         fmm_case("fmm_tiny", 2, 2.0, 0.7, t2[:40], n2[:40], s2[:40], sl2[:40], dl2[:40], tg2)
+    if want("hs_fault_p2"):
+        hs_case("hs_fault_p2", 2)
+    if want("hs_fault_p1"):
+        hs_case("hs_fault_p1", 1)
     if want("fmm_fault"):
         fmm_case("fmm_fault", 2, 1.0, 0.8, *fault_case())
     for name, iprec in (("iter_c1_p2", 2), ("iter_c1_p1", 1)):

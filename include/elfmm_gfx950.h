@@ -152,6 +152,28 @@ void elh3dtriadirecttarg_(const double *rlam, const double *rmu, const double *t
  * reference's routine for that part. No counterpart in the reference. */
 void elfmm_gfx950_half_parts(int mask);
 
+/* el3dtriadirecttarg on device arrays without any allocation: no counterpart
+ * in the reference; for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments and results as el3dtriadirecttarg_ (device
+ * pointers, Fortran layouts), enqueued on `stream` (a hipStream_t, NULL for
+ * the default stream); the call returns at once and the outputs are ready
+ * when the stream has run it. The workspace is a static 46 MiB device array
+ * held by every device the library runs on until the process ends:
+ * nsource <= 131072 and nsource + ntarget <= 262144, else it returns 128
+ * and enqueues nothing. All calls on one device share that array: calls
+ * that overlap on one device (different streams, or threads) race. Returns
+ * 0, 128 (sizes above the limits) or 32 (HIP error). */
+int elfmm_gfx950_el3dtriadirecttarg_noalloc(double rlam, double rmu, int nsource,
+                                            const double *d_triangle, const double *d_trinorm,
+                                            const double *d_source, int ifsingle,
+                                            const double *d_sigma_sl, int ifdouble,
+                                            const double *d_sigma_dl, int ifptfrc, double *d_ptfrc,
+                                            int ifstrain, double *d_strain, int ntarget,
+                                            const double *d_target, int ifptfrctarg,
+                                            double *d_ptfrctarg, int ifstraintarg,
+                                            double *d_straintarg, void *stream);
+
 /* Device-resident elfmm3dtriatarg: all arrays are device pointers with the
  * Fortran layouts above; the call is enqueued on `stream` (a hipStream_t,
  * NULL for the default stream) and returns after the outputs are written.

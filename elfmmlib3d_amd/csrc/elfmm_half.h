@@ -37,4 +37,47 @@ int mindlin_far_dev(int iprec, double rlam, double rmu, int n, const double *d_t
                     const double *d_tgt, int ifptt, double *d_ptt, int ifstt, double *d_stt, hipStream_t stream);
 hipError_t launch_mirror(int n, const double *in, double *out, hipStream_t st);
 
+// ---- Mindlin harmonic far field (elfmm_mindlin.hip) ----
+struct MindArgs {
+    int p, ifsl, ifdl;
+    double lam, mu;
+    double tsize;                    // root box side
+    const int *level, *src_start, *src_count;
+    const double *centers;
+    const double *tri, *nrm, *ssl, *sdl;   // triangles of the Mindlin tree's sources (mirrored), tree order
+    int nc;                          // doubles per expansion (ncoef(p))
+    double *mp, *lo;                 // [box][4][nc]
+};
+
+// Plane-wave discretisation on the device (per physical sample j: lambda,
+// cos u, sin u, unused; per lambda: first Fourier mode / sample, counts).
+struct MindPw {
+    int nlams, nexptot, nexptotp;
+    const double *tab;        // per physical sample j: lam, cos u, sin u, 1/np
+    const int *jnl;           // lambda index of physical sample j
+    const int *foff, *poff;   // per lambda: first Fourier mode / physical sample
+    const int *nfour, *nphys;
+    const double *rlams, *whts;
+};
+
+struct MindDev {
+    MindArgs args;
+    MindPw pw;
+    const int *slot;              // per box: plane-wave / proxy slot (-1: none)
+    const double *ycst;           // ylgndr(p, 1/sqrt(3))
+    int nq, nquad;                // proxy charges per ring; sphere quadrature size
+    double *pwo, *skel;           // outgoing plane waves, proxy charges
+    const double *gl, *ytab, *eph, *wl;
+};
+hipError_t launch_mind_p2m(const MindDev &d, const int *boxes, int nbx, hipStream_t st);
+hipError_t launch_mind_bscale(const MindDev &d, const int *boxes, int nbx, hipStream_t st);
+hipError_t launch_mind_bmod(const MindDev &d, const int *boxes, int nbx, hipStream_t st);
+hipError_t launch_mind_proxy(const MindDev &d, const int *tb, const int *ts, const int *ti, int nbx, hipStream_t st);
+hipError_t launch_mind_far(const MindDev &d, const int *tb, const int *ts, const int *ti, int nbx, hipStream_t st);
+hipError_t launch_mind_assemble(int n, int nt, const int *izt, const double *pts, const double *lap, double mu,
+                                int ifpt, double *ptt, int ifst, double *stt, hipStream_t st);
+size_t mind_proxy_lds(int p, int nquad, int nq);
+size_t mind_far_lds(int nexptotp, int nexptot);
+void mind_set_lds_limits(size_t proxy, size_t far);
+
 }  // namespace elfmm

@@ -140,4 +140,23 @@ constexpr int EW_PTS = 32;                           // points per evaluation wa
 void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool direct_only, int nrhs,
                 Plan &P, bool iter = false);
 
+// Task lists of the Mindlin harmonic far field (lfmm3d_list2_mindlin,
+// src/lfmm3droutsmindlin.f:28-529) on its own octree, beside the Laplace
+// plan of the same tree (P2M boxes, M2M/L2L launches, L2T tasks).
+struct MindlinPlan {
+    std::vector<int> src_boxes;                // boxes of level >= 2 with sources
+    std::vector<int> b_scale_boxes[64];        // per level: boxes with sources and kids (B rescaled after M2M)
+    // per target box of level >= 2 with targets: far list-2 sources (|dz| >=
+    // 1.5 sides: plane waves), near ones (proxy-charge local); CSR
+    std::vector<int> far_boxes, far_start, far_items;
+    std::vector<int> near_boxes, near_start, near_items;
+    GemmLaunch m2l;                            // near pairs, grouped by offset (full order)
+    std::vector<M2LTile> m2l_tiles;
+    std::vector<int> pw_slot;                  // per box: plane-wave slot (-1: none)
+    int npw = 0;
+};
+// ops: translation operators of full order p for the near offsets
+// (|dz| <= 1) of list 2.
+void build_mindlin_plan(const Tree &T, const Operators &ops, MindlinPlan &MP);
+
 }  // namespace elfmm

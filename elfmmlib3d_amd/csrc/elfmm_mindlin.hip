@@ -1,15 +1,727 @@
-// Mindlin harmonic-image far field (lfmm3dmindlintriatarg,
-// src/lfmm3dtriamindlin_iter_1.f:41) on gfx950.
+// Mindlin harmonic-image far field on gfx950 (lfmm3dmindlintriatarg,
+// src/lfmm3dtriamindlin_iter_1.f:41; passes src/lfmm3droutsmindlin.f:28).
+// The kernels here are the steps that are not plain Laplace translations;
+// M2M, L2L, the near list-2 M2L and L2T run on the Laplace kernels of
+// elfmm_gpu.hip with the three expansions B, C1, C2 as right-hand sides.
+// One workgroup per box (or per target box) throughout: the per-box work is
+// O(p^3) with p = 26 and the boxes are few (leaves of up to 240 sources).
 #include <hip/hip_runtime.h>
 
 #include "elfmm_half.h"
+#include "elfmm_internal.h"
+#include "elfmm_mindlin.h"
 
 namespace elfmm {
 
-int mindlin_far_dev(int, double, double, int, const double *, const double *, const double *, int, const double *,
-                    int, const double *, int, const double *, int, double *, int, double *, hipStream_t)
+constexpr int MW = 256;                      // threads per workgroup
+constexpr int MP_MAX = 39;                   // largest supported order (iprec 3)
+constexpr int MH = (MP_MAX + 1) * (MP_MAX + 2) / 2;
+constexpr int MPW_FOUR = 544;                // Fourier modes of the largest plane-wave table (525)
+
+__constant__ double c_mrule[12][3] = {ELFMM_RULE6};
+
+// Regular harmonics R_l^m(x) (Racah normalisation, Condon-Shortley phase)
+// of one point for l <= L, 0 <= m <= l, into H[pk(l, m)] (re, im): thread m
+// computes column m. Caller synchronises.
+__device__ void harm_regular(int L, double x, double y, double z, double2 *H)
 {
-    return 16;   // not available yet
+    int m = threadIdx.x;
+    if (m > L) return;
+    double r2 = x * x + y * y + z * z;
+    double dr = 1.0, di = 0.0;
+    for (int k = 1; k <= m; k++) {
+        double f = sqrt((2.0 * k - 1) / (2.0 * k));
+        double nr = -(x * dr - y * di) * f, ni = -(x * di + y * dr) * f;
+        dr = nr;
+        di = ni;
+    }
+    double ar = dr, ai = di, br = 0, bi = 0;
+    H[pk(m, m)] = make_double2(ar, ai);
+    if (m + 1 <= L) {
+        double f = sqrt(2.0 * m + 1) * z;
+        br = f * ar;
+        bi = f * ai;
+        H[pk(m + 1, m)] = make_double2(br, bi);
+    }
+    for (int l = m + 2; l <= L; l++) {
+        double f1 = (2.0 * l - 1) * z, f2 = sqrt((l - 1.0 + m) * (l - 1.0 - m)) * r2, d = sqrt((l + m) * (double)(l - m));
+        double cr = (f1 * br - f2 * ar) / d, ci = (f1 * bi - f2 * ai) / d;
+        H[pk(l, m)] = make_double2(cr, ci);
+        ar = br;
+        ai = bi;
+        br = cr;
+        bi = ci;
+    }
+}
+
+// R_l^m for any m (R^{-m} = (-1)^m conj(R^m)); zero outside 0 <= |m| <= l.
+__device__ __forceinline__ double2 hget(const double2 *H, int l, int m)
+{
+    if (l < 0 || m > l || -m > l) return make_double2(0.0, 0.0);
+    if (m >= 0) return H[pk(l, m)];
+    double2 c = H[pk(l, -m)];
+    return (m & 1) ? make_double2(-c.x, c.y) : make_double2(c.x, -c.y);
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Ladders of the regular harmonics: d_z R_l^m = sqrt((l+m)(l-m)) R_{l-1}^m,
+// (d_x + i d_y) R_l^m = sqrt((l-m)(l-m-1)) R_{l-1}^{m+1},
+// (d_x - i d_y) R_l^m = -sqrt((l+m)(l+m-1)) R_{l-1}^{m-1}.
+__device__ __forceinline__ double lz(int l, int m) { return (l + m) * (double)(l - m) > 0 ? sqrt((l + m) * (double)(l - m)) : 0.0; }
+__device__ __forceinline__ double lp(int l, int m)
+{
+    double v = (l - m) * (double)(l - m - 1);
+    return v > 0 ? sqrt(v) : 0.0;
+}
+__device__ __forceinline__ double lmn(int l, int m)
+{
+    double v = (l + m) * (double)(l + m - 1);
+    return v > 0 ? -sqrt(v) : 0.0;
+}
+
+// Source operator of one quadrature node for one expansion: charge q,
+// dipole d.grad, quadrupole sum Q_ab d_a d_b (xx yy zz xy xz yz; each pair
+// once), acting on the source position.
+struct SrcOp {
+    double q, d[3], Q[6];
+};
+
+// (D R_l^m)(x) for the operator D, from the harmonics table H of x.
+__device__ double2 apply_op(const SrcOp &S, const double2 *H, int l, int m)
+{
+    double2 acc = make_double2(0.0, 0.0);
+    if (S.q != 0.0) {
+        double2 r = hget(H, l, m);
+        acc.x += S.q * r.x;
+        acc.y += S.q * r.y;
+    }
+    // d.grad = dz * d_z + ((dx - i dy) d_+ + (dx + i dy) d_-) / 2
+    if (S.d[0] != 0.0 || S.d[1] != 0.0 || S.d[2] != 0.0) {
+        double2 rz = hget(H, l - 1, m), rp = hget(H, l - 1, m + 1), rm = hget(H, l - 1, m - 1);
+        double z = S.d[2] * lz(l, m);
+        acc.x += z * rz.x;
+        acc.y += z * rz.y;
+        double2 cp = make_double2(0.5 * S.d[0] * lp(l, m), -0.5 * S.d[1] * lp(l, m));
+        double2 cm = make_double2(0.5 * S.d[0] * lmn(l, m), 0.5 * S.d[1] * lmn(l, m));
+        double2 a = cmul(cp, rp), b = cmul(cm, rm);
+        acc.x += a.x + b.x;
+        acc.y += a.y + b.y;
+    }
+    // sum Q d d = A d_+^2 + B d_-^2 + C d_z^2 + E d_+ d_z + F d_- d_z
+    // (d_+ d_- = -d_z^2 on harmonic functions)
+    bool quad = false;
+    for (int k = 0; k < 6; k++) quad |= S.Q[k] != 0.0;
+    if (quad) {
+        double2 A = make_double2((S.Q[0] - S.Q[1]) / 4, -S.Q[3] / 4);
+        double2 B = make_double2((S.Q[0] - S.Q[1]) / 4, S.Q[3] / 4);
+        double C = S.Q[2] - (S.Q[0] + S.Q[1]) / 2;
+        double2 E = make_double2(S.Q[4] / 2, -S.Q[5] / 2);
+        double2 F = make_double2(S.Q[4] / 2, S.Q[5] / 2);
+        // d_+^2 R_l^m = lp(l,m) lp(l-1,m+1) R_{l-2}^{m+2}
+        double2 t;
+        t = hget(H, l - 2, m + 2);
+        double c = lp(l, m) * lp(l - 1, m + 1);
+        double2 u = cmul(A, make_double2(c * t.x, c * t.y));
+        acc.x += u.x;
+        acc.y += u.y;
+        t = hget(H, l - 2, m - 2);
+        c = lmn(l, m) * lmn(l - 1, m - 1);
+        u = cmul(B, make_double2(c * t.x, c * t.y));
+        acc.x += u.x;
+        acc.y += u.y;
+        t = hget(H, l - 2, m);
+        c = C * lz(l, m) * lz(l - 1, m);
+        acc.x += c * t.x;
+        acc.y += c * t.y;
+        // d_+ d_z R_l^m = lz(l,m) lp(l-1,m) R_{l-2}^{m+1}
+        t = hget(H, l - 2, m + 1);
+        c = lz(l, m) * lp(l - 1, m);
+        u = cmul(E, make_double2(c * t.x, c * t.y));
+        acc.x += u.x;
+        acc.y += u.y;
+        t = hget(H, l - 2, m - 1);
+        c = lz(l, m) * lmn(l - 1, m);
+        u = cmul(F, make_double2(c * t.x, c * t.y));
+        acc.x += u.x;
+        acc.y += u.y;
+    }
+    return acc;
+}
+
+
+// P2M of B, C1, C2 from the quadrature nodes of a leaf's (mirrored)
+// triangles (l3dtriaformmp_mindlinb_trunc / _mindlinc_trunc,
+// src/mindlinformmptri.f:305, :449 with formmp_mindlinb_trunc
+// src/mindlinbformmp.f and formmp_mindlinc_trunc src/mindlincformmp.f):
+// nodes of the order-6 rule with charge and dipole densities times
+// weight * 2 * area; B: dipole (strength box side, vector sigma_sl) and a
+// quadrupole of the double layer; C1: dipole and charge of sigma_sl,
+// quadrupole and dipole of the double layer; C2: dipole and quadrupole.
+__global__ __launch_bounds__(MW) void k_mind_p2m(MindArgs a, const int *__restrict__ boxes)
+{
+    __shared__ double2 H[MH];
+    int b = boxes[blockIdx.x];
+    int p = a.p;
+    double side = a.tsize / (double)(1 << a.level[b]);
+    const double *c = a.centers + 3 * (size_t)b;
+    double al = (a.lam + a.mu) / (a.lam + 2 * a.mu);
+    int nown = (p + 1) * (p + 2) / 2;
+    double2 acc[3][2];
+    for (int e = 0; e < 3; e++)
+        for (int k = 0; k < 2; k++) acc[e][k] = make_double2(0.0, 0.0);
+    int s0 = a.src_start[b], sn = a.src_count[b];
+    for (int ti = s0; ti < s0 + sn; ti++) {
+        const double *t = a.tri + 9 * (size_t)ti;
+        double ax = t[3] - t[0], ay = t[4] - t[1], az = t[5] - t[2];
+        double bx = t[6] - t[0], by = t[7] - t[1], bz = t[8] - t[2];
+        double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+        double area = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+        const double *nv = a.nrm + 3 * (size_t)ti;
+        for (int qn = 0; qn < 12; qn++) {
+            double u = c_mrule[qn][0], v = c_mrule[qn][1];
+            double X[3];
+            for (int d = 0; d < 3; d++) X[d] = t[d] + u * (t[3 + d] - t[d]) + v * (t[6 + d] - t[d]);
+            double w = c_mrule[qn][2] * area * 2;
+            SrcOp S[3];
+            for (int e = 0; e < 3; e++) {
+                S[e].q = 0;
+                for (int k = 0; k < 3; k++) S[e].d[k] = 0;
+                for (int k = 0; k < 6; k++) S[e].Q[k] = 0;
+            }
+            double y3 = X[2];
+            if (a.ifsl) {
+                double sg[3] = {a.ssl[3 * (size_t)ti] * w, a.ssl[3 * (size_t)ti + 1] * w, a.ssl[3 * (size_t)ti + 2] * w};
+                double s0v[3] = {sg[0], sg[1], -sg[2]};
+                for (int k = 0; k < 3; k++) {
+                    S[0].d[k] += side * sg[k];
+                    S[1].d[k] += -al * y3 * s0v[k];
+                    S[2].d[k] += -(2 - al) * s0v[k];
+                }
+                S[1].q += (2 - al) * s0v[2];
+            }
+            if (a.ifdl) {
+                double bv[3] = {a.sdl[3 * (size_t)ti] * w, a.sdl[3 * (size_t)ti + 1] * w, a.sdl[3 * (size_t)ti + 2] * w};
+                double q1 = bv[0] * nv[0], q2 = bv[1] * nv[1], q3 = bv[2] * nv[2];
+                double q4 = bv[0] * nv[1] + bv[1] * nv[0];
+                double q5 = bv[0] * nv[2] + bv[2] * nv[0], q6 = bv[1] * nv[2] + bv[2] * nv[1];
+                double rf = 2.0 * a.mu * side, rr = (q1 + q2 + q3) * side;
+                S[0].Q[0] += q1 * rf;
+                S[0].Q[1] += q2 * rf;
+                S[0].Q[2] += -q3 * rf - a.lam * rr * 2.0;
+                S[0].Q[3] += q4 * rf;
+                double r1 = 2.0 * al * y3 * a.mu;
+                S[1].Q[0] += -r1 * q1;
+                S[1].Q[1] += -r1 * q2;
+                S[1].Q[2] += -r1 * q3;
+                S[1].Q[3] += -r1 * q4;
+                S[1].Q[4] += r1 * q5;
+                S[1].Q[5] += r1 * q6;
+                double r2 = (2.0 - 2.0 * al) * a.mu;
+                S[1].d[0] += -r2 * q5;
+                S[1].d[1] += -r2 * q6;
+                S[1].d[2] += r2 * (q3 + q3) - a.lam * 2.0 * (al - 1) * (q1 + q2 + q3);
+                double r3 = 2.0 * (2.0 - al) * a.mu;
+                S[2].Q[0] += -r3 * q1;
+                S[2].Q[1] += -r3 * q2;
+                S[2].Q[2] += -r3 * q3;
+                S[2].Q[3] += -r3 * q4;
+                S[2].Q[4] += r3 * q5;
+                S[2].Q[5] += r3 * q6;
+            }
+            __syncthreads();
+            harm_regular(p, X[0] - c[0], X[1] - c[1], X[2] - c[2], H);
+            __syncthreads();
+            for (int k = 0; k < 2; k++) {
+                int idx = threadIdx.x + k * MW;
+                if (idx >= nown) break;
+                int l = 0;
+                while ((l + 1) * (l + 2) / 2 <= idx) l++;
+                int m = idx - l * (l + 1) / 2;
+                for (int e = 0; e < 3; e++) {
+                    double2 v = apply_op(S[e], H, l, m);
+                    acc[e][k].x += v.x;
+                    acc[e][k].y -= v.y;          // the multipole takes conj(D R)
+                }
+            }
+        }
+    }
+    for (int k = 0; k < 2; k++) {
+        int idx = threadIdx.x + k * MW;
+        if (idx >= nown) break;
+        int l = 0;
+        while ((l + 1) * (l + 2) / 2 <= idx) l++;
+        double sc = pow(side, -(double)l);
+        for (int e = 0; e < 3; e++) {
+            double *o = a.mp + ((size_t)b * 4 + e) * a.nc + 2 * idx;
+            o[0] += acc[e][k].x * sc;
+            o[1] += acc[e][k].y * sc;
+        }
+    }
+}
+
+// B of boxes with sources and kids times 2 after their M2M (mprstemp with
+// the ratio of the children's and the parent's scales,
+// src/lfmm3droutsmindlin.f:206-210).
+__global__ void k_mind_bscale(const int *__restrict__ boxes, int n, int nc, double *mp)
+{
+    int i = blockIdx.x;
+    if (i >= n) return;
+    double *o = mp + (size_t)boxes[i] * 4 * nc;
+    for (int k = threadIdx.x; k < nc; k += blockDim.x) o[k] *= 2.0;
+}
+
+
+// Per source box of level >= 2: the outgoing plane waves of B, C1, C2
+// (mpoletoexp + ftophys; B scaled by (1-alpha)/alpha * side / lambda^2),
+// B's proxy charges (mptoslpborder) and B replaced by its remainder after
+// the proxies' multipole, integrated twice in z and times
+// (1-alpha)/alpha * side, the form the near list-2 M2L uses
+// (src/lfmm3droutsmindlin.f:257-279, mindlinbmploc src/mindlinmpeval.f:320).
+// pwo: [slot][3][nexptotp] complex; skel: [slot][2 nq][4] (x, y, z, q);
+// ycst: ylgndr(p, 1/sqrt(3)), (p+1)^2.
+__global__ __launch_bounds__(MW) void k_mind_bmod(MindArgs a, MindPw pw, const int *__restrict__ boxes,
+                                                  const int *__restrict__ slot, const double *__restrict__ ycst,
+                                                  int nq, double *__restrict__ pwo, double *__restrict__ skel)
+{
+    __shared__ double2 H[MH];
+    __shared__ double2 F[3][MPW_FOUR];
+    __shared__ double2 ch[2][MP_MAX + 1];
+    int b = boxes[blockIdx.x];
+    int p = a.p, nown = (p + 1) * (p + 2) / 2;
+    double side = a.tsize / (double)(1 << a.level[b]);
+    const double *c = a.centers + 3 * (size_t)b;
+    double al = (a.lam + a.mu) / (a.lam + 2 * a.mu);
+    double *Mb = a.mp + (size_t)b * 4 * a.nc;
+    double *po = pwo + (size_t)slot[b] * 3 * 2 * pw.nexptotp;
+    // Fourier modes of the three expansions: f_{nl,m} = (-i)^m sum_n (-lam)^n / sqrt((n-m)!(n+m)!) M_n^m
+    for (int t = threadIdx.x; t < 3 * pw.nexptot; t += MW) {
+        int e = t / pw.nexptot, f = t % pw.nexptot;
+        int nl = 0;
+        while (nl + 1 < pw.nlams && pw.foff[nl + 1] <= f) nl++;
+        int mth = f - pw.foff[nl];
+        double lam = pw.rlams[nl];
+        double s1r = 0, s1i = 0, s2r = 0, s2i = 0;
+        const double *M = Mb + (size_t)e * a.nc;
+        for (int n = mth; n <= p; n++) {
+            double cc = pow(-lam, (double)n) / (sqrt_fact(n - mth) * sqrt_fact(n + mth));
+            const double *Mm = M + 2 * pk(n, mth);
+            if (((n - mth) & 1) == 0) {
+                s1r += cc * Mm[0];
+                s1i += cc * Mm[1];
+            } else {
+                s2r += cc * Mm[0];
+                s2i += cc * Mm[1];
+            }
+        }
+        double sr = s1r + s2r, si = s1i + s2i;
+        double zr, zi;
+        switch (mth & 3) {
+        case 0: zr = 1; zi = 0; break;
+        case 1: zr = 0; zi = -1; break;
+        case 2: zr = -1; zi = 0; break;
+        default: zr = 0; zi = 1; break;
+        }
+        F[e][f] = make_double2(sr * zr - si * zi, sr * zi + si * zr);
+    }
+    __syncthreads();
+    // physical samples
+    double scb = (1.0 - al) / al * side;
+    for (int t = threadIdx.x; t < 3 * pw.nexptotp; t += MW) {
+        int e = t / pw.nexptotp, j = t % pw.nexptotp;
+        int nl = pw.jnl[j];
+        int np = pw.nphys[nl], nf = pw.nfour[nl];
+        double alpha = (j - pw.poff[nl]) * (2 * M_PI / np);
+        const double2 *f = &F[e][pw.foff[nl]];
+        double pr = f[0].x, pi = f[0].y;
+        double sgn = -1.0;
+        for (int mm = 1; mm < nf; mm++) {
+            double cr = cos(mm * alpha), ci = sin(mm * alpha);
+            double tr = cr * f[mm].x - ci * f[mm].y, ti = cr * f[mm].y + ci * f[mm].x;
+            pr += tr + sgn * tr;
+            pi += ti - sgn * ti;
+            sgn = -sgn;
+        }
+        if (e == 0) {
+            double z = 1.0 / (pw.rlams[nl] * pw.rlams[nl]);
+            pr = pr * z * scb;
+            pi = pi * z * scb;
+        }
+        po[2 * ((size_t)e * pw.nexptotp + j)] = pr;
+        po[2 * ((size_t)e * pw.nexptotp + j) + 1] = pi;
+    }
+    // proxy rings of B's sectoral terms
+    const double ct = 1.0 / sqrt(3.0), st = sqrt(2.0 / 3.0);
+    double radius = side * sqrt(3.0) / 2;
+    if ((int)threadIdx.x <= p) {
+        int j = threadIdx.x;
+        double rfac = 1.0 / (radius / side);           // 1 / (radius * rscale)
+        double rmulj = pow(rfac, (double)j), rmulj1 = rmulj * rfac;
+        const double *Mjj = Mb + 2 * pk(j, j);
+        double d = ycst[j * (p + 1) + j] * st;
+        double f1r = Mjj[0] / d * rmulj * sqrt(2 * j + 1.0), f1i = Mjj[1] / d * rmulj * sqrt(2 * j + 1.0);
+        double f2r = 0, f2i = 0;
+        if (j + 1 <= p) {
+            const double *M2 = Mb + 2 * pk(j + 1, j);
+            double d2 = ycst[(j + 1) * (p + 1) + j] * st;
+            f2r = M2[0] / d2 * rmulj1 * sqrt(2 * (j + 1) + 1.0);
+            f2i = M2[1] / d2 * rmulj1 * sqrt(2 * (j + 1) + 1.0);
+        }
+        ch[0][j] = make_double2((f1r + f2r) / 2, (f1i + f2i) / 2);
+        ch[1][j] = make_double2((f1r - f2r) / 2, (f1i - f2i) / 2);
+    }
+    __syncthreads();
+    double *sk = skel + (size_t)slot[b] * 2 * nq * 4;
+    for (int j = threadIdx.x; j < 2 * nq; j += MW) {
+        int ring = j / nq, jj = j % nq;
+        double phi = jj * (2 * M_PI / nq);
+        double s = ch[ring][0].x;
+        for (int k = 1; k <= p; k++) s += 2 * (ch[ring][k].x * cos(k * phi) - ch[ring][k].y * sin(k * phi));
+        sk[4 * j] = c[0] + radius * st * cos(phi);
+        sk[4 * j + 1] = c[1] + radius * st * sin(phi);
+        sk[4 * j + 2] = c[2] + (ring ? -radius * ct : radius * ct);
+        sk[4 * j + 3] = s * st / nq;
+    }
+    __threadfence_block();
+    __syncthreads();
+    // multipole of the proxy charges
+    double2 acc[2] = {make_double2(0, 0), make_double2(0, 0)};
+    for (int j = 0; j < 2 * nq; j++) {
+        double x = sk[4 * j] - c[0], y = sk[4 * j + 1] - c[1], z = sk[4 * j + 2] - c[2], q = sk[4 * j + 3];
+        __syncthreads();
+        harm_regular(p, x, y, z, H);
+        __syncthreads();
+        for (int k = 0; k < 2; k++) {
+            int idx = threadIdx.x + k * MW;
+            if (idx >= nown) break;
+            double2 r = H[idx];
+            acc[k].x += q * r.x;
+            acc[k].y -= q * r.y;
+        }
+    }
+    // remainder of B (scaled proxies' multipole subtracted), dz^-2, scale
+    __syncthreads();
+    for (int k = 0; k < 2; k++) {
+        int idx = threadIdx.x + k * MW;
+        if (idx >= nown) break;
+        int l = 0;
+        while ((l + 1) * (l + 2) / 2 <= idx) l++;
+        double sc = pow(side, -(double)l);
+        H[idx] = make_double2(Mb[2 * idx] - acc[k].x * sc, Mb[2 * idx + 1] - acc[k].y * sc);
+    }
+    __syncthreads();
+    for (int k = 0; k < 2; k++) {
+        int idx = threadIdx.x + k * MW;
+        if (idx >= nown) break;
+        int l = 0;
+        while ((l + 1) * (l + 2) / 2 <= idx) l++;
+        int m = idx - l * (l + 1) / 2;
+        double vr = 0, vi = 0;
+        if (l <= p - 2) {
+            int i = l + 2;
+            double rs = 1.0 / sqrt((i - m) * (i - m - 1.0) * (i + m) * (i + m - 1.0));
+            double2 v = H[pk(i, m)];
+            vr = v.x * rs * scb;
+            vi = v.y * rs * scb;
+        }
+        Mb[2 * idx] = vr;
+        Mb[2 * idx + 1] = vi;
+    }
+}
+
+// Per target box: the U-field of the proxy rings of each near list-2 source
+// box, tabulated on the sphere about the target centre and projected on
+// the local expansion of B (l3dformtaintkerbf90sc, src/mindlinformta.f:980).
+// gl: Gauss-Legendre nodes (nquad) then weights (nquad); ytab: ylgndr at the
+// nodes, [nquad][(p+1)^2]; eph: e^{-i m 2 pi (k+1)/nquad}, [m][nquad].
+__global__ __launch_bounds__(MW) void k_mind_proxy(MindArgs a, const int *__restrict__ tboxes,
+                                                   const int *__restrict__ tstart, const int *__restrict__ titems,
+                                                   const int *__restrict__ slot, const double *__restrict__ skel,
+                                                   int nq, int nquad, const double *__restrict__ gl,
+                                                   const double *__restrict__ ytab, const double2 *__restrict__ eph)
+{
+    extern __shared__ double smem[];
+    double *phi = smem;                                   // nquad * nquad
+    double2 *marr = (double2 *)(smem + nquad * nquad);    // nquad * (p + 1)
+    double *sk = (double *)(marr + nquad * (a.p + 1));    // 2 nq * 4
+    int t = tboxes[blockIdx.x];
+    int p = a.p, nown = (p + 1) * (p + 2) / 2;
+    double side = a.tsize / (double)(1 << a.level[t]);
+    const double *c = a.centers + 3 * (size_t)t;
+    double al = (a.lam + a.mu) / (a.lam + 2 * a.mu);
+    double radius = side * sqrt(3.0) / 2;
+    double rs = side / radius;                            // 1 / (rscale * radius)
+    double2 acc[2] = {make_double2(0, 0), make_double2(0, 0)};
+    for (int it = tstart[blockIdx.x]; it < tstart[blockIdx.x + 1]; it++) {
+        int s = titems[it];
+        double sside = a.tsize / (double)(1 << a.level[s]);
+        double sc = (1.0 - al) / al / sside;              // times the source box's scale
+        const double *ks = skel + (size_t)slot[s] * 2 * nq * 4;
+        __syncthreads();
+        for (int k = threadIdx.x; k < 2 * nq * 4; k += MW) sk[k] = ks[k];
+        __syncthreads();
+        for (int g = threadIdx.x; g < nquad * nquad; g += MW) {
+            int jj = g / nquad, kk = g % nquad;
+            double cth = gl[jj], sth = sqrt(1.0 - cth * cth);
+            double ph = 2 * M_PI * (kk + 1) / nquad;
+            double tx = c[0] + radius * sth * cos(ph), ty = c[1] + radius * sth * sin(ph), tz = c[2] + radius * cth;
+            double v = 0.0;
+            for (int i = 0; i < 2 * nq; i++) {
+                double r1 = tx - sk[4 * i], r2 = ty - sk[4 * i + 1], r3 = tz - sk[4 * i + 2];
+                double rr = sqrt(r1 * r1 + r2 * r2 + r3 * r3);
+                v += sk[4 * i + 3] * (-r3 * log(rr - r3) - rr);
+            }
+            phi[g] = v * sc;
+        }
+        __syncthreads();
+        for (int g = threadIdx.x; g < nquad * (p + 1); g += MW) {
+            int jj = g / (p + 1), m = g % (p + 1);
+            double sr = 0, si = 0;
+            for (int kk = 0; kk < nquad; kk++) {
+                double2 e = eph[m * nquad + kk];
+                sr += phi[jj * nquad + kk] * e.x;
+                si += phi[jj * nquad + kk] * e.y;
+            }
+            marr[jj * (p + 1) + m] = make_double2(sr / (2 * nquad) * gl[nquad + jj], si / (2 * nquad) * gl[nquad + jj]);
+        }
+        __syncthreads();
+        for (int k = 0; k < 2; k++) {
+            int idx = threadIdx.x + k * MW;
+            if (idx >= nown) break;
+            int l = 0;
+            while ((l + 1) * (l + 2) / 2 <= idx) l++;
+            int m = idx - l * (l + 1) / 2;
+            double f0 = sqrt(2 * l + 1.0) * pow(rs, (double)l);
+            double sr = 0, si = 0;
+            for (int jj = 0; jj < nquad; jj++) {
+                double y = ytab[(size_t)jj * (p + 1) * (p + 1) + l * (p + 1) + m];
+                double2 mv = marr[jj * (p + 1) + m];
+                sr += mv.x * y;
+                si += mv.y * y;
+            }
+            acc[k].x += sr * f0;
+            acc[k].y += si * f0;
+        }
+    }
+    for (int k = 0; k < 2; k++) {
+        int idx = threadIdx.x + k * MW;
+        if (idx >= nown) break;
+        double *o = a.lo + (size_t)t * 4 * a.nc + 2 * idx;
+        o[0] += acc[k].x;
+        o[1] += acc[k].y;
+    }
+}
+
+// Per target box: the plane waves of its far list-2 sources shifted and
+// summed (xyzshift_add; a box that also has sources starts from its own
+// outgoing waves, as the reference's shared array does), then back to
+// Fourier modes (phystof) and to the local expansions of B, C1, C2
+// (exptolocald_add). wl: per (lambda, degree) whts * lam^n, [nl][p+1].
+__global__ __launch_bounds__(MW) void k_mind_far(MindArgs a, MindPw pw, const int *__restrict__ tboxes,
+                                                 const int *__restrict__ tstart, const int *__restrict__ titems,
+                                                 const int *__restrict__ slot, const double *__restrict__ pwo,
+                                                 const double *__restrict__ wl)
+{
+    extern __shared__ double smem[];
+    double2 *acc = (double2 *)smem;                       // [3][nexptotp]
+    double2 *F = acc + 3 * pw.nexptotp;                   // [3][nexptot]
+    int t = tboxes[blockIdx.x];
+    int p = a.p, nown = (p + 1) * (p + 2) / 2;
+    double side = a.tsize / (double)(1 << a.level[t]);
+    const double *c = a.centers + 3 * (size_t)t;
+    int own = slot[t];
+    for (int k = threadIdx.x; k < 3 * pw.nexptotp; k += MW) {
+        if (own >= 0) {
+            const double *s = pwo + (size_t)own * 3 * 2 * pw.nexptotp + 2 * k;
+            acc[k] = make_double2(s[0], s[1]);
+        } else {
+            acc[k] = make_double2(0.0, 0.0);
+        }
+    }
+    for (int it = tstart[blockIdx.x]; it < tstart[blockIdx.x + 1]; it++) {
+        int s = titems[it];
+        double sside = a.tsize / (double)(1 << a.level[s]);
+        const double *cs = a.centers + 3 * (size_t)s;
+        double xd = (c[0] - cs[0]) / sside, yd = (c[1] - cs[1]) / sside, zd = (c[2] - cs[2]) / sside;
+        const double *src = pwo + (size_t)slot[s] * 3 * 2 * pw.nexptotp;
+        for (int j = threadIdx.x; j < pw.nexptotp; j += MW) {
+            const double *tb = pw.tab + 4 * j;
+            double lam = tb[0];
+            double r0 = exp(lam * zd);
+            double ph = lam * (xd * tb[1] + yd * tb[2]);
+            double mr = r0 * cos(ph), mi = r0 * sin(ph);
+            for (int e = 0; e < 3; e++) {
+                double x = src[2 * (e * pw.nexptotp + j)], y = src[2 * (e * pw.nexptotp + j) + 1];
+                double2 &o = acc[e * pw.nexptotp + j];
+                o.x += x * mr - y * mi;
+                o.y += x * mi + y * mr;
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 3 * pw.nexptot; k += MW) {
+        int e = k / pw.nexptot, f = k % pw.nexptot;
+        int nl = 0;
+        while (nl + 1 < pw.nlams && pw.foff[nl + 1] <= f) nl++;
+        int m = f - pw.foff[nl], np = pw.nphys[nl];
+        double fr = 0, fi = 0;
+        for (int j = 0; j < np; j++) {
+            double2 v = acc[e * pw.nexptotp + pw.poff[nl] + j];
+            if (m == 0) {
+                fr += v.x;
+                fi += v.y;
+            } else {
+                double al = j * (2 * M_PI / np);
+                double cr = cos(m * al), ci = -sin(m * al);
+                fr += cr * v.x - ci * v.y;
+                fi += cr * v.y + ci * v.x;
+            }
+        }
+        F[k] = make_double2(fr / np, fi / np);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 3 * nown; k += MW) {
+        int e = k / nown, idx = k % nown;
+        int n = 0;
+        while ((n + 1) * (n + 2) / 2 <= idx) n++;
+        int m = idx - n * (n + 1) / 2;
+        double ar = 0, ai = 0;
+        for (int nl = 0; nl < pw.nlams; nl++) {
+            if (m > pw.nfour[nl] - 1) continue;
+            double2 f = F[e * pw.nexptot + pw.foff[nl] + m];
+            double r = wl[nl * (p + 1) + n];
+            ar += r * f.x;
+            ai += r * f.y;
+        }
+        double zr, zi;
+        switch (m & 3) {
+        case 0: zr = 1; zi = 0; break;
+        case 1: zr = 0; zi = -1; break;
+        case 2: zr = -1; zi = 0; break;
+        default: zr = 0; zi = 1; break;
+        }
+        double sc = 1.0 / (sqrt_fact(n - m) * sqrt_fact(n + m)) / side;
+        double *o = a.lo + ((size_t)t * 4 + e) * a.nc + 2 * idx;
+        o[0] += (ar * zr - ai * zi) * sc;
+        // The m = 0 coefficients come out complex. Their imaginary parts
+        // carry only the imaginary part of the potential, which the
+        // reference drops when it returns the real displacement and strain;
+        // the packed (m >= 0) storage assumes a real potential, so they are
+        // dropped here.
+        if (m > 0) o[1] += (ar * zi + ai * zr) * sc;
+    }
+}
+
+// Targets: displacement and strain of the Mindlin harmonic part from the
+// L2T fields of B, C1, C2 (lap: per point 4 x (pot, grad, xx yy zz xy xz yz))
+// (mindlintaevalb/c_trunc, src/mindlinformta.f:830-978; combined as
+// src/lfmm3dtriamindlin_iter_1.f:1130-1169 and added as
+// src/elhfmm3dtria_iter.f:806-831: displacement + pot/mu, strain -
+// sym(field)/(2 mu)).
+__global__ void k_mind_assemble(int n, int nt, const int *__restrict__ izt, const double *__restrict__ pts,
+                                const double *__restrict__ lap, double mu, int ifpt, double *__restrict__ ptt,
+                                int ifst, double *__restrict__ stt)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nt) return;
+    int ip = n + j;
+    const double *f = lap + (size_t)ip * 40;
+    double z = pts[3 * (size_t)ip + 2];
+    // B: fld = (d1 pot, d2 pot, -d3 pot)
+    const double *B = f, *C1 = f + 10, *C2 = f + 20;
+    double fb[3] = {B[1], B[2], -B[3]};
+    double hb[6] = {B[4], B[5], B[6], B[7], B[8], B[9]};   // xx yy zz xy xz yz
+    double gb[3][3] = {{-hb[0], -hb[3], hb[4]}, {-hb[3], -hb[1], hb[5]}, {-hb[4], -hb[5], hb[2]}};
+    // C: fld = grad pot1 - (0, 0, pot2)
+    double fc[3] = {C1[1], C1[2], C1[3] - C2[0]};
+    double hc[6] = {C1[4], C1[5], C1[6], C1[7], C1[8], C1[9]};
+    double f2[3] = {-C2[1], -C2[2], -C2[3]};               // -grad pot2
+    double gc[3][3] = {{-hc[0], -hc[3], -hc[4] - f2[0]},
+                       {-hc[3], -hc[1], -hc[5] - f2[1]},
+                       {-hc[4], -hc[5], -hc[2] - f2[2]}};
+    double u[3], H[3][3];
+    for (int k = 0; k < 3; k++) u[k] = fb[k] + z * fc[k];
+    for (int k = 0; k < 3; k++)
+        for (int l = 0; l < 3; l++) H[k][l] = gb[k][l] + z * gc[k][l];
+    for (int l = 0; l < 3; l++) H[2][l] -= fc[l];
+    int o = izt[j];
+    if (ifpt)
+        for (int k = 0; k < 3; k++) ptt[3 * (size_t)o + k] += u[k] / mu;
+    if (ifst)
+        for (int k = 0; k < 3; k++)
+            for (int l = 0; l < 3; l++) stt[9 * (size_t)o + 3 * l + k] -= (H[k][l] + H[l][k]) / (2 * mu);
+}
+
+// ---- launchers ----
+
+hipError_t launch_mind_p2m(const MindDev &d, const int *boxes, int nbx, hipStream_t st)
+{
+    if (nbx <= 0) return hipSuccess;
+    const MindArgs &a = d.args;
+    hipLaunchKernelGGL(k_mind_p2m, dim3(nbx), dim3(MW), 0, st, a, boxes);
+    return hipGetLastError();
+}
+
+hipError_t launch_mind_bscale(const MindDev &d, const int *boxes, int nbx, hipStream_t st)
+{
+    if (nbx <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mind_bscale, dim3(nbx), dim3(256), 0, st, boxes, nbx, d.args.nc, d.args.mp);
+    return hipGetLastError();
+}
+
+hipError_t launch_mind_bmod(const MindDev &d, const int *boxes, int nbx, hipStream_t st)
+{
+    if (nbx <= 0) return hipSuccess;
+    const MindArgs &a = d.args;
+    const MindPw &pw = d.pw;
+    hipLaunchKernelGGL(k_mind_bmod, dim3(nbx), dim3(MW), 0, st, a, pw, boxes, d.slot, d.ycst, d.nq, d.pwo, d.skel);
+    return hipGetLastError();
+}
+
+hipError_t launch_mind_proxy(const MindDev &d, const int *tb, const int *ts, const int *ti, int nbx, hipStream_t st)
+{
+    if (nbx <= 0) return hipSuccess;
+    const MindArgs &a = d.args;
+    size_t sh = (size_t)d.nquad * d.nquad * 8 + (size_t)d.nquad * (a.p + 1) * 16 + (size_t)2 * d.nq * 4 * 8;
+    hipLaunchKernelGGL(k_mind_proxy, dim3(nbx), dim3(MW), sh, st, a, tb, ts, ti, d.slot, d.skel, d.nq, d.nquad, d.gl,
+                       d.ytab, (const double2 *)d.eph);
+    return hipGetLastError();
+}
+
+hipError_t launch_mind_far(const MindDev &d, const int *tb, const int *ts, const int *ti, int nbx, hipStream_t st)
+{
+    if (nbx <= 0) return hipSuccess;
+    const MindArgs &a = d.args;
+    const MindPw &pw = d.pw;
+    size_t sh = (size_t)3 * (pw.nexptotp + pw.nexptot) * 16;
+    hipLaunchKernelGGL(k_mind_far, dim3(nbx), dim3(MW), sh, st, a, pw, tb, ts, ti, d.slot, d.pwo, d.wl);
+    return hipGetLastError();
+}
+
+hipError_t launch_mind_assemble(int n, int nt, const int *izt, const double *pts, const double *lap, double mu,
+                                int ifpt, double *ptt, int ifst, double *stt, hipStream_t st)
+{
+    if (nt <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mind_assemble, dim3((nt + 255) / 256), dim3(256), 0, st, n, nt, izt, pts, lap, mu, ifpt, ptt,
+                       ifst, stt);
+    return hipGetLastError();
+}
+
+size_t mind_proxy_lds(int p, int nquad, int nq)
+{
+    return (size_t)nquad * nquad * 8 + (size_t)nquad * (p + 1) * 16 + (size_t)2 * nq * 4 * 8;
+}
+
+size_t mind_far_lds(int nexptotp, int nexptot) { return (size_t)3 * (nexptotp + nexptot) * 16; }
+
+void mind_set_lds_limits(size_t proxy, size_t far)
+{
+    (void)hipFuncSetAttribute((const void *)k_mind_proxy, hipFuncAttributeMaxDynamicSharedMemorySize, (int)proxy);
+    (void)hipFuncSetAttribute((const void *)k_mind_far, hipFuncAttributeMaxDynamicSharedMemorySize, (int)far);
+    (void)hipGetLastError();
 }
 
 }  // namespace elfmm

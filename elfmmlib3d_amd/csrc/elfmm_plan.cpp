@@ -452,4 +452,72 @@ void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool 
     P.stats[13] = (double)P.l2t_boxes.size();
 }
 
+void build_mindlin_plan(const Tree &T, const Operators &ops, MindlinPlan &MP)
+{
+    MP = MindlinPlan();
+    int nb = T.nboxes;
+    MP.pw_slot.assign(nb, -1);
+    for (int b = 0; b < nb; b++) {
+        if (T.level[b] < 2 || !T.src_count[b]) continue;
+        MP.src_boxes.push_back(b);
+        MP.pw_slot[b] = MP.npw++;
+        if (T.nkids[b] && T.level[b] < 64) MP.b_scale_boxes[T.level[b]].push_back(b);
+    }
+    std::vector<int> cnt(343, 0);
+    std::vector<std::pair<int, int>> near_pairs;   // (offset, pair index)
+    std::vector<int> ns, nd;
+    MP.far_start.push_back(0);
+    MP.near_start.push_back(0);
+    for (int b = 0; b < nb; b++) {
+        if (T.level[b] < 2 || !T.tgt_count[b]) continue;
+        int nfar = 0, nnear = 0;
+        for (int q = T.lstart[2][b]; q < T.lstart[2][b + 1]; q++) {
+            int j = T.litems[2][q];
+            if (!T.src_count[j]) continue;
+            // z offset of target and source centres in box sides (src/lfmm3droutsmindlin.f:329-333)
+            double side = T.size / (double)(1 << T.level[j]);
+            double zdis = (T.centers[3 * b + 2] - T.centers[3 * j + 2]) / side;
+            if (std::fabs(zdis) < 1.5) {
+                MP.near_items.push_back(j);
+                nnear++;
+                int o = offset_index(T, j, b);
+                near_pairs.push_back({o, (int)ns.size()});
+                ns.push_back(j);
+                nd.push_back(b);
+            } else {
+                MP.far_items.push_back(j);
+                nfar++;
+            }
+        }
+        if (nfar) {
+            MP.far_boxes.push_back(b);
+            MP.far_start.push_back((int)MP.far_items.size());
+        }
+        if (nnear) {
+            MP.near_boxes.push_back(b);
+            MP.near_start.push_back((int)MP.near_items.size());
+        }
+    }
+    // near pairs grouped by offset, tiles of the full-order operators
+    std::stable_sort(near_pairs.begin(), near_pairs.end(),
+                     [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first < b.first; });
+    GemmLaunch &g = MP.m2l;
+    for (size_t k = 0; k < near_pairs.size();) {
+        size_t e = k;
+        while (e < near_pairs.size() && near_pairs[e].first == near_pairs[k].first) e++;
+        int o = near_pairs[k].first;
+        std::vector<int> s(e - k), d(e - k);
+        std::vector<double> sc(e - k);
+        for (size_t i = k; i < e; i++) {
+            s[i - k] = ns[near_pairs[i].second];
+            d[i - k] = nd[near_pairs[i].second];
+            sc[i - k] = (double)(1 << T.level[d[i - k]]) / T.size;
+        }
+        int h = ncoef(ops.m2l_pt[o]) / 2;
+        add_pairs(g, ops.m2l_off[o], h, ops.m2l_phi[o], s.data(), d.data(), sc.data(), (int)(e - k));
+        k = e;
+    }
+    MP.m2l_tiles = g.tiles;
+}
+
 }  // namespace elfmm

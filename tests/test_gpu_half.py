@@ -74,3 +74,47 @@ def test_direct_sum(gpu_lib):
     sub = g["direct_idx"]
     _, _, pt, st = gpu_lib.elh3dtriadirecttarg(lam, mu, tri, nrm, src, ssl, sdl, tg[sub])
     assert rel(pt, g["direct_pt"]) < 1e-11 and rel(st, g["direct_st"]) < 1e-11
+
+
+def test_mindlin_far_field(case):
+    E, g = case
+    ier, _, pt, st = run(E, g, 4)
+    assert ier == 0
+    assert rel(pt, g["mind_pt"]) < TOL and rel(st, g["mind_st"]) < TOL, (rel(pt, g["mind_pt"]),
+                                                                         rel(st, g["mind_st"]))
+
+
+def test_half_space_fmm(case):
+    """The whole image part: icomp 1, then the store (icomp 2) and replay
+    (icomp 3) cycle, against the reference's own parts (its icomp 2 and 3
+    near field integrates the unit densities)."""
+    E, g = case
+    ier, _, pt, st = run(E, g, 7)
+    assert ier == 0
+    assert rel(pt, g["full_pt"]) < TOL and rel(st, g["full_st"]) < TOL, (rel(pt, g["full_pt"]),
+                                                                         rel(st, g["full_st"]))
+    want_pt = g["anti_pt"] + g["mind_pt"] + g["near2_pt"]
+    want_st = g["anti_st"] + g["mind_st"] + g["near2_st"]
+    ws = np.zeros(4)
+    for icomp in (2, 3):
+        ier, _, pt, st = run(E, g, 7, icomp=icomp, wsave=ws)
+        assert ier == 0
+        assert rel(pt, want_pt) < TOL and rel(st, want_st) < TOL, (icomp, rel(pt, want_pt), rel(st, want_st))
+    assert E.iter_release(ws) == 1
+
+
+def test_half_space_fmm_vs_direct(gpu_lib):
+    """Direct arrival (elfmm3dtriatargiter) + image part (elhfmm3dtriatargiter)
+    against the GPU half-space direct sum, within the requested tolerance
+    (iprec 2: 0.5e-6 for displacement, one digit more for strain)."""
+    E = gpu_lib
+    g = load_golden("hs_fault_p2")
+    lam, mu, tri, nrm, src, ssl, sdl, tg = inputs(g)
+    sub = g["direct_idx"]
+    ier, _, _, _, pa, sa = E.elfmm3dtriatargiter(2, lam, mu, tri, nrm, src, ssl, sdl, tg[sub], ifptfrc=0,
+                                                 ifstrain=0)
+    assert ier == 0
+    ier, _, ph, sh = E.elhfmm3dtriatargiter(2, lam, mu, tri, nrm, src, ssl, sdl, tg[sub])
+    assert ier == 0
+    _, _, pd, sd = E.elh3dtriadirecttarg(lam, mu, tri, nrm, src, ssl, sdl, tg[sub])
+    assert rel(pa + ph, pd) < 0.5e-6 and rel(sa + sh, sd) < 0.5e-5, (rel(pa + ph, pd), rel(sa + sh, sd))

@@ -47,6 +47,7 @@ struct MindArgs {
     const double *tri, *nrm, *ssl, *sdl;   // triangles of the Mindlin tree's sources (mirrored), tree order
     int nc;                          // doubles per expansion (ncoef(p))
     double *mp, *lo;                 // [box][4][nc]
+    int nqtri;                       // order of the P2M triangle rule: 2, 4 or 6 (3, 6 or 12 nodes)
 };
 
 // Plane-wave discretisation on the device (per physical sample j: lambda,

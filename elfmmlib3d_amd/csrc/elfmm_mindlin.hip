@@ -18,7 +18,8 @@ constexpr int MP_MAX = 39;                   // largest supported order (iprec 3
 constexpr int MH = (MP_MAX + 1) * (MP_MAX + 2) / 2;
 constexpr int MPW_FOUR = 544;                // Fourier modes of the largest plane-wave table (525)
 
-__constant__ double c_mrule[12][3] = {ELFMM_RULE6};
+// rows 0-2: order 2, rows 3-8: order 4, rows 9-20: order 6
+__constant__ double c_mrule[21][3] = {ELFMM_RULE2, ELFMM_RULE4, ELFMM_RULE6};
 
 // Regular harmonics R_l^m(x) (Racah normalisation, Condon-Shortley phase)
 // of one point for l <= L, 0 <= m <= l, into H[pk(l, m)] (re, im): thread m
@@ -157,8 +158,9 @@ __device__ double2 apply_op(const SrcOp &S, const double2 *H, int l, int m)
 // triangles (l3dtriaformmp_mindlinb_trunc / _mindlinc_trunc,
 // src/mindlinformmptri.f:305, :449 with formmp_mindlinb_trunc
 // src/mindlinbformmp.f and formmp_mindlinc_trunc src/mindlincformmp.f):
-// nodes of the order-6 rule with charge and dipole densities times
-// weight * 2 * area; B: dipole (strength box side, vector sigma_sl) and a
+// nodes of the order-nqtri rule (order 6 from iprec 1 up, 4 at iprec 0,
+// 2 below, src/lfmm3dtriamindlin_iter_1.f:679-693) with charge and dipole
+// densities times weight * 2 * area; B: dipole (strength box side, vector sigma_sl) and a
 // quadrupole of the double layer; C1: dipole and charge of sigma_sl,
 // quadrupole and dipole of the double layer; C2: dipole and quadrupole.
 __global__ __launch_bounds__(MW) void k_mind_p2m(MindArgs a, const int *__restrict__ boxes)
@@ -174,6 +176,7 @@ __global__ __launch_bounds__(MW) void k_mind_p2m(MindArgs a, const int *__restri
     for (int e = 0; e < 3; e++)
         for (int k = 0; k < 2; k++) acc[e][k] = make_double2(0.0, 0.0);
     int s0 = a.src_start[b], sn = a.src_count[b];
+    int q0 = a.nqtri == 2 ? 0 : a.nqtri == 4 ? 3 : 9, q1 = a.nqtri == 2 ? 3 : a.nqtri == 4 ? 9 : 21;
     for (int ti = s0; ti < s0 + sn; ti++) {
         const double *t = a.tri + 9 * (size_t)ti;
         double ax = t[3] - t[0], ay = t[4] - t[1], az = t[5] - t[2];
@@ -181,7 +184,7 @@ __global__ __launch_bounds__(MW) void k_mind_p2m(MindArgs a, const int *__restri
         double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
         double area = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
         const double *nv = a.nrm + 3 * (size_t)ti;
-        for (int qn = 0; qn < 12; qn++) {
+        for (int qn = q0; qn < q1; qn++) {
             double u = c_mrule[qn][0], v = c_mrule[qn][1];
             double X[3];
             for (int d = 0; d < 3; d++) X[d] = t[d] + u * (t[3 + d] - t[d]) + v * (t[6 + d] - t[d]);

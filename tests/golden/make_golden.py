@@ -161,6 +161,25 @@ def hs_case(name, iprec, lam=1.0, mu=0.8):
          sigma_sl2=ssl2, sigma_dl2=sdl2, target=tg, direct_idx=sub, **out)
 
 
+def hs_low_case(name, lam=1.1, mu=0.9):
+    """The half-space apply (icomp 1) and its Mindlin part at iprec 0 and -1,
+    where the Mindlin P2M uses the 6- and 3-node triangle rules."""
+    tri, nrm, src, ssl, sdl, _, _, tg = hs_geometry(seed=5, ntri=900, ntarg=80)
+    out = {}
+    for tag, iprec in (("p0", 0), ("m1", -1)):
+        ier, lused, _, _, _ = ref.elhfmm3dtriatargiter(iprec, lam, mu, tri, nrm, src, ssl, sdl, tg, icomp_type=4)
+        assert ier == 0
+        ws = np.zeros(lused + 16)
+        ier, _, out[f"full_{tag}_pt"], out[f"full_{tag}_st"], _ = ref.elhfmm3dtriatargiter(
+            iprec, lam, mu, tri, nrm, src, ssl, sdl, tg, icomp_type=1, wsave=ws)
+        assert ier == 0
+        ier, out[f"mind_{tag}_pt"], out[f"mind_{tag}_st"] = ref.lfmm3d_mindlin(iprec, lam, mu, tri, nrm, src, ssl,
+                                                                              sdl, tg)
+        assert ier == 0
+    save(name, lam=lam, mu=mu, triangle=tri, trinorm=nrm, source=src, sigma_sl=ssl, sigma_dl=sdl, target=tg,
+         **out)
+
+
 def main(only=None):
     want = lambda name: only is None or name in only
     tri, nrm, src, ssl, sdl, tg = sphere_case(6, 300, 11)
@@ -186,6 +205,8 @@ def main(only=None):
         hs_case("hs_fault_p2", 2)
     if want("hs_fault_p1"):
         hs_case("hs_fault_p1", 1)
+    if want("hs_fault_low"):
+        hs_low_case("hs_fault_low")
     if want("fmm_fault"):
         fmm_case("fmm_fault", 2, 1.0, 0.8, *fault_case())
     for name, iprec in (("iter_c1_p2", 2), ("iter_c1_p1", 1)):

@@ -1,8 +1,10 @@
-"""Drop-in check from Fortran: tests/fortran/drop_in.f90 calls
+"""Drop-in checks from Fortran: tests/fortran/drop_in.f90 calls
 elfmm3dtriatarg through an implicit interface with the argument list of
-src/elfmm3dtria.f:162-167 and is linked against libelfmm_gfx950.so in place
-of the reference's objects. Its outputs must match the reference FMM's on
-the same inputs (relative L2 <= 1e-10)."""
+src/elfmm3dtria.f:162-167, tests/fortran/drop_in_half.f90 calls
+elhfmm3dtriatargiter (src/elhfmm3dtria_iter.f:46-51) and
+elh3dtriadirecttarg (src/elh3dtria.f:53-58); both are linked against
+libelfmm_gfx950.so in place of the reference's objects. Their outputs must
+match the reference's on the same inputs (relative L2 <= 1e-10)."""
 import os
 import subprocess
 
@@ -38,3 +40,41 @@ def test_fortran_caller_matches_reference(tmp_path, case):
     parts = np.split(v, np.cumsum(sizes)[:-1])
     for a, k in zip(parts, ("ptfrc", "strain", "ptfrctarg", "straintarg")):
         assert rel(a, g[k].ravel()) < 1e-10, k
+
+
+EXE_H = os.path.join(ROOT, "tests", "fortran", "_build", "drop_in_half")
+
+
+@pytest.mark.skipif(not os.path.exists(EXE_H), reason="Fortran driver not built (no flang when build() ran)")
+@pytest.mark.parametrize("case", ["hs_fault_p2", "hs_fault_p1"])
+def test_fortran_half_space_caller_matches_reference(tmp_path, case):
+    """tests/fortran/drop_in_half.f90: elhfmm3dtriatargiter through the iter
+    protocol (4, 2, 3, then 1) and elh3dtriadirecttarg at a subset of the
+    targets, against the reference's outputs (its icomp 2 and 3 results are
+    the anti-image and Mindlin far fields plus its icomp-2 near field)."""
+    g = load_golden(case)
+    n, nt = g["triangle"].shape[0], g["target"].shape[0]
+    sub = g["direct_idx"]
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        np.array([n, nt, int(g["iprec"]), len(sub)], dtype=np.int32).tofile(f)
+        np.array([float(g["lam"]), float(g["mu"])]).tofile(f)
+        for k in ("triangle", "trinorm", "source", "sigma_sl", "sigma_dl", "target"):
+            np.ascontiguousarray(g[k], dtype=np.float64).tofile(f)
+        np.ascontiguousarray(g["target"][sub], dtype=np.float64).tofile(f)
+    print(f"case {case}: fortran drop_in_half n={n} nt={nt}", flush=True)
+    r = subprocess.run([EXE_H, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    raw = open(fout, "rb").read()
+    ier4, lused, ier2, ier3, ier1 = (int(x) for x in np.frombuffer(raw[:20], dtype=np.int32))
+    assert (ier4, ier2, ier3, ier1) == (0, 0, 0, 0) and lused > 0
+    v = np.frombuffer(raw[20:], dtype=np.float64)
+    sizes = [3 * nt, 9 * nt] * 3 + [3 * len(sub), 9 * len(sub)]
+    parts = np.split(v, np.cumsum(sizes)[:-1])
+    want23_pt = (g["anti_pt"] + g["mind_pt"] + g["near2_pt"]).ravel()
+    want23_st = (g["anti_st"] + g["mind_st"] + g["near2_st"]).ravel()
+    for (a, b), (w, k) in zip(zip(parts[0:6:2], parts[1:6:2]),
+                              [((want23_pt, want23_st), 2), ((want23_pt, want23_st), 3),
+                               ((g["full_pt"].ravel(), g["full_st"].ravel()), 1)]):
+        assert rel(a, w[0]) < 1e-10 and rel(b, w[1]) < 1e-10, (k, rel(a, w[0]), rel(b, w[1]))
+    assert rel(parts[6], g["direct_pt"].ravel()) < 1e-11 and rel(parts[7], g["direct_st"].ravel()) < 1e-11

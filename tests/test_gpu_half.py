@@ -103,6 +103,28 @@ def test_half_space_fmm(case):
     assert E.iter_release(ws) == 1
 
 
+@pytest.mark.parametrize("tag,iprec", [("p0", 0), ("m1", -1)])
+def test_half_space_low_precision(gpu_lib, tag, iprec):
+    """iprec 0 and -1: the Mindlin P2M on the 6- and 3-node triangle rules,
+    smaller trees (nbox 60 and 30) and the lowest plane-wave table."""
+    E = gpu_lib
+    g = load_golden("hs_fault_low")
+    args = (float(g["lam"]), float(g["mu"]), g["triangle"], g["trinorm"], g["source"], g["sigma_sl"],
+            g["sigma_dl"], g["target"])
+    try:
+        E.half_parts(4)
+        ier, _, pt, st = E.elhfmm3dtriatargiter(iprec, *args)
+        assert ier == 0
+        assert rel(pt, g[f"mind_{tag}_pt"]) < TOL and rel(st, g[f"mind_{tag}_st"]) < TOL, (
+            rel(pt, g[f"mind_{tag}_pt"]), rel(st, g[f"mind_{tag}_st"]))
+    finally:
+        E.half_parts(7)
+    ier, _, pt, st = E.elhfmm3dtriatargiter(iprec, *args)
+    assert ier == 0
+    assert rel(pt, g[f"full_{tag}_pt"]) < TOL and rel(st, g[f"full_{tag}_st"]) < TOL, (
+        rel(pt, g[f"full_{tag}_pt"]), rel(st, g[f"full_{tag}_st"]))
+
+
 def test_half_space_fmm_vs_direct(gpu_lib):
     """Direct arrival (elfmm3dtriatargiter) + image part (elhfmm3dtriatargiter)
     against the GPU half-space direct sum, within the requested tolerance

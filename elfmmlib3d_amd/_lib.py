@@ -231,5 +231,5 @@ def last_timings():
     out = np.zeros(16)
     library().elfmm_gfx950_last_timings(_p(out))
     keys = ["plan", "upload", "p2m", "m2m", "m2l", "p2l", "l2l", "l2t", "m2t", "near",
-            "assemble", "total", "tree", "plan_tasks"]
+            "assemble", "total", "tree", "plan_tasks", "mindlin", "half_total"]
     return dict(zip(keys, out.tolist()))

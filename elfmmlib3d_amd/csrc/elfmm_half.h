@@ -69,7 +69,10 @@ struct MindDev {
     int nq, nquad;                // proxy charges per ring; sphere quadrature size
     double *pwo, *skel;           // outgoing plane waves, proxy charges
     const double *gl, *ytab, *eph, *wl;
+    double *gmom;                 // P2M moments, mind_p2m_scratch_bytes
 };
+// scratch of launch_mind_p2m for nbx boxes
+size_t mind_p2m_scratch_bytes(int p, int nbx);
 hipError_t launch_mind_p2m(const MindDev &d, const int *boxes, int nbx, hipStream_t st);
 hipError_t launch_mind_bscale(const MindDev &d, const int *boxes, int nbx, hipStream_t st);
 hipError_t launch_mind_bmod(const MindDev &d, const int *boxes, int nbx, hipStream_t st);

@@ -7,6 +7,8 @@
 // O(p^3) with p = 26 and the boxes are few (leaves of up to 240 sources).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "elfmm_half.h"
 #include "elfmm_internal.h"
 #include "elfmm_mindlin.h"
@@ -17,6 +19,7 @@ constexpr int MW = 256;                      // threads per workgroup
 constexpr int MP_MAX = 39;                   // largest supported order (iprec 3)
 constexpr int MH = (MP_MAX + 1) * (MP_MAX + 2) / 2;
 constexpr int MPW_FOUR = 544;                // Fourier modes of the largest plane-wave table (525)
+constexpr size_t MIND_P2M_LDS = 100 * 1024;   // LDS of the P2M moment pass
 
 // rows 0-2: order 2, rows 3-8: order 4, rows 9-20: order 6
 __constant__ double c_mrule[21][3] = {ELFMM_RULE2, ELFMM_RULE4, ELFMM_RULE6};
@@ -84,185 +87,245 @@ __device__ __forceinline__ double lmn(int l, int m)
     return v > 0 ? -sqrt(v) : 0.0;
 }
 
-// Source operator of one quadrature node for one expansion: charge q,
-// dipole d.grad, quadrupole sum Q_ab d_a d_b (xx yy zz xy xz yz; each pair
-// once), acting on the source position.
-struct SrcOp {
-    double q, d[3], Q[6];
-};
-
-// (D R_l^m)(x) for the operator D, from the harmonics table H of x.
-__device__ double2 apply_op(const SrcOp &S, const double2 *H, int l, int m)
+// Column m (l = m..L) of the regular harmonics of one point into H[pk(l, m)].
+__device__ void harm_column(int L, int m, double x, double y, double z, double2 *H)
 {
-    double2 acc = make_double2(0.0, 0.0);
-    if (S.q != 0.0) {
-        double2 r = hget(H, l, m);
-        acc.x += S.q * r.x;
-        acc.y += S.q * r.y;
+    double r2 = x * x + y * y + z * z;
+    double dr = 1.0, di = 0.0;
+    for (int k = 1; k <= m; k++) {
+        double f = sqrt((2.0 * k - 1) / (2.0 * k));
+        double nr = -(x * dr - y * di) * f, ni = -(x * di + y * dr) * f;
+        dr = nr;
+        di = ni;
     }
-    // d.grad = dz * d_z + ((dx - i dy) d_+ + (dx + i dy) d_-) / 2
-    if (S.d[0] != 0.0 || S.d[1] != 0.0 || S.d[2] != 0.0) {
-        double2 rz = hget(H, l - 1, m), rp = hget(H, l - 1, m + 1), rm = hget(H, l - 1, m - 1);
-        double z = S.d[2] * lz(l, m);
-        acc.x += z * rz.x;
-        acc.y += z * rz.y;
-        double2 cp = make_double2(0.5 * S.d[0] * lp(l, m), -0.5 * S.d[1] * lp(l, m));
-        double2 cm = make_double2(0.5 * S.d[0] * lmn(l, m), 0.5 * S.d[1] * lmn(l, m));
-        double2 a = cmul(cp, rp), b = cmul(cm, rm);
-        acc.x += a.x + b.x;
-        acc.y += a.y + b.y;
+    double ar = dr, ai = di, br = 0, bi = 0;
+    H[pk(m, m)] = make_double2(ar, ai);
+    if (m + 1 > L) return;
+    double f = sqrt(2.0 * m + 1) * z;
+    br = f * ar;
+    bi = f * ai;
+    H[pk(m + 1, m)] = make_double2(br, bi);
+    for (int l = m + 2; l <= L; l++) {
+        double f1 = (2.0 * l - 1) * z, f2 = sqrt((l - 1.0 + m) * (l - 1.0 - m)) * r2, d = sqrt((l + m) * (double)(l - m));
+        double cr = (f1 * br - f2 * ar) / d, ci = (f1 * bi - f2 * ai) / d;
+        H[pk(l, m)] = make_double2(cr, ci);
+        ar = br;
+        ai = bi;
+        br = cr;
+        bi = ci;
     }
-    // sum Q d d = A d_+^2 + B d_-^2 + C d_z^2 + E d_+ d_z + F d_- d_z
-    // (d_+ d_- = -d_z^2 on harmonic functions)
-    bool quad = false;
-    for (int k = 0; k < 6; k++) quad |= S.Q[k] != 0.0;
-    if (quad) {
-        double2 A = make_double2((S.Q[0] - S.Q[1]) / 4, -S.Q[3] / 4);
-        double2 B = make_double2((S.Q[0] - S.Q[1]) / 4, S.Q[3] / 4);
-        double C = S.Q[2] - (S.Q[0] + S.Q[1]) / 2;
-        double2 E = make_double2(S.Q[4] / 2, -S.Q[5] / 2);
-        double2 F = make_double2(S.Q[4] / 2, S.Q[5] / 2);
-        // d_+^2 R_l^m = lp(l,m) lp(l-1,m+1) R_{l-2}^{m+2}
-        double2 t;
-        t = hget(H, l - 2, m + 2);
-        double c = lp(l, m) * lp(l - 1, m + 1);
-        double2 u = cmul(A, make_double2(c * t.x, c * t.y));
-        acc.x += u.x;
-        acc.y += u.y;
-        t = hget(H, l - 2, m - 2);
-        c = lmn(l, m) * lmn(l - 1, m - 1);
-        u = cmul(B, make_double2(c * t.x, c * t.y));
-        acc.x += u.x;
-        acc.y += u.y;
-        t = hget(H, l - 2, m);
-        c = C * lz(l, m) * lz(l - 1, m);
-        acc.x += c * t.x;
-        acc.y += c * t.y;
-        // d_+ d_z R_l^m = lz(l,m) lp(l-1,m) R_{l-2}^{m+1}
-        t = hget(H, l - 2, m + 1);
-        c = lz(l, m) * lp(l - 1, m);
-        u = cmul(E, make_double2(c * t.x, c * t.y));
-        acc.x += u.x;
-        acc.y += u.y;
-        t = hget(H, l - 2, m - 1);
-        c = lz(l, m) * lmn(l - 1, m);
-        u = cmul(F, make_double2(c * t.x, c * t.y));
-        acc.x += u.x;
-        acc.y += u.y;
+}
+
+// P2M of B, C1, C2 (l3dtriaformmp_mindlinb_trunc / _mindlinc_trunc,
+// src/mindlinformmptri.f:305, :449 with formmp_mindlinb_trunc
+// src/mindlinbformmp.f and formmp_mindlinc_trunc src/mindlincformmp.f) from
+// the nodes of the order-nqtri rule on a leaf's (mirrored) triangles (order
+// 6 from iprec 1 up, 4 at iprec 0, 2 below,
+// src/lfmm3dtriamindlin_iter_1.f:679-693), densities times weight * 2 *
+// area. Every charge, dipole and quadrupole strength the three expansions
+// take at a node is a fixed combination of 18 real columns:
+//   0-2 sigma_sl, 3-5 y3 sigma_sl, 6-11 q, 12-17 y3 q,
+// q = (b1 n1, b2 n2, b3 n3, b1 n2 + b2 n1, b1 n3 + b3 n1, b2 n3 + b3 n2)
+// with b = sigma_dl and n the original normal, y3 the node's (mirrored)
+// height. Pass 1 sums, per box and column, the column times the regular
+// harmonics of the node about the box centre (moments G_col(l, m)); pass 2
+// forms each coefficient from the moments through the harmonics' ladders.
+
+constexpr int MCOLS = 18;
+
+// Pass 1: one workgroup per box, nch nodes at a time: node columns and
+// harmonics in LDS, then thread task (coefficient, group of 6 columns)
+// accumulates over the nodes. NT >= (p+1)(p+2)/2.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_mind_moments(MindArgs a, const int *__restrict__ boxes, int nch,
+                                                     double2 *__restrict__ G)
+{
+    extern __shared__ double smem[];
+    int p = a.p, nown = (p + 1) * (p + 2) / 2;
+    double2 *Hs = (double2 *)smem;                       // [nch][nown]
+    double *Ss = (double *)(Hs + (size_t)nch * nown);    // [nch][MCOLS]
+    double *Xs = Ss + nch * MCOLS;                       // [nch][3], relative to the centre
+    int b = boxes[blockIdx.x];
+    const double *c = a.centers + 3 * (size_t)b;
+    int q0 = a.nqtri == 2 ? 0 : a.nqtri == 4 ? 3 : 9, q1 = a.nqtri == 2 ? 3 : a.nqtri == 4 ? 9 : 21;
+    int nqn = q1 - q0;
+    int s0 = a.src_start[b];
+    int K = a.src_count[b] * nqn;
+    int tid = threadIdx.x, ntask = 3 * nown;
+    double2 acc[3][6];
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int k = 0; k < 6; k++) acc[j][k] = make_double2(0.0, 0.0);
+    for (int k0 = 0; k0 < K; k0 += nch) {
+        int nk = min(nch, K - k0);
+        __syncthreads();
+        if (tid < nk) {
+            int ti = s0 + (k0 + tid) / nqn, qn = q0 + (k0 + tid) % nqn;
+            const double *t = a.tri + 9 * (size_t)ti;
+            double ax = t[3] - t[0], ay = t[4] - t[1], az = t[5] - t[2];
+            double bx = t[6] - t[0], by = t[7] - t[1], bz = t[8] - t[2];
+            double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+            double area = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+            double u = c_mrule[qn][0], v = c_mrule[qn][1];
+            double X[3];
+            for (int d = 0; d < 3; d++) X[d] = t[d] + u * (t[3 + d] - t[d]) + v * (t[6 + d] - t[d]);
+            double w = c_mrule[qn][2] * area * 2, y3 = X[2];
+            double *s = Ss + tid * MCOLS;
+            for (int k = 0; k < 3; k++) {
+                double sg = a.ifsl ? a.ssl[3 * (size_t)ti + k] * w : 0.0;
+                s[k] = sg;
+                s[3 + k] = y3 * sg;
+            }
+            double q[6] = {0, 0, 0, 0, 0, 0};
+            if (a.ifdl) {
+                const double *nv = a.nrm + 3 * (size_t)ti;
+                double bv[3] = {a.sdl[3 * (size_t)ti] * w, a.sdl[3 * (size_t)ti + 1] * w, a.sdl[3 * (size_t)ti + 2] * w};
+                q[0] = bv[0] * nv[0];
+                q[1] = bv[1] * nv[1];
+                q[2] = bv[2] * nv[2];
+                q[3] = bv[0] * nv[1] + bv[1] * nv[0];
+                q[4] = bv[0] * nv[2] + bv[2] * nv[0];
+                q[5] = bv[1] * nv[2] + bv[2] * nv[1];
+            }
+            for (int k = 0; k < 6; k++) {
+                s[6 + k] = q[k];
+                s[12 + k] = y3 * q[k];
+            }
+            for (int d = 0; d < 3; d++) Xs[3 * tid + d] = X[d] - c[d];
+        }
+        __syncthreads();
+        for (int task = tid; task < nk * (p + 1); task += NT) {
+            int k = task / (p + 1), m = task % (p + 1);
+            harm_column(p, m, Xs[3 * k], Xs[3 * k + 1], Xs[3 * k + 2], Hs + (size_t)k * nown);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            int t = tid + j * NT;
+            if (t < ntask) {
+                int coef = t % nown, g = t / nown;
+                for (int k = 0; k < nk; k++) {
+                    double2 v = Hs[(size_t)k * nown + coef];
+                    const double *s = Ss + k * MCOLS + 6 * g;
+#pragma unroll
+                    for (int jj = 0; jj < 6; jj++) {
+                        acc[j][jj].x += s[jj] * v.x;
+                        acc[j][jj].y += s[jj] * v.y;
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        int t = tid + j * NT;
+        if (t < ntask) {
+            int coef = t % nown, g = t / nown;
+#pragma unroll
+            for (int jj = 0; jj < 6; jj++)
+                G[((size_t)blockIdx.x * MCOLS + 6 * g + jj) * nown + coef] = acc[j][jj];
+        }
+    }
+}
+
+// Moment of column col at (l, m), any m (G^{-m} = (-1)^m conj(G^m): the
+// columns are real); zero outside 0 <= |m| <= l.
+__device__ __forceinline__ double2 gmom(const double2 *G, int nown, int col, int l, int m)
+{
+    if (l < 0 || m > l || -m > l) return make_double2(0.0, 0.0);
+    double2 v = G[(size_t)col * nown + pk(l, m < 0 ? -m : m)];
+    if (m >= 0) return v;
+    return (m & 1) ? make_double2(-v.x, v.y) : make_double2(v.x, -v.y);
+}
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 cscl(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+// a - i b and a + i b
+__device__ __forceinline__ double2 cmi(double2 a, double2 b) { return make_double2(a.x + b.y, a.y - b.x); }
+__device__ __forceinline__ double2 cpi(double2 a, double2 b) { return make_double2(a.x - b.y, a.y + b.x); }
+
+// (D R_l^m) summed over the nodes for the operator D with charge moment Mq,
+// dipole moments Mx, My, Mz and quadrupole moments Q0..Q5 (xx yy zz xy xz
+// yz, each pair once), each a function (l', m') -> complex moment:
+//   d.grad = d_z d_z + ((d_x - i d_y) d_+ + (d_x + i d_y) d_-) / 2,
+//   sum Q d d = A d_+^2 + B d_-^2 + C d_z^2 + E d_+ d_z + F d_- d_z
+// (d_+ d_- = -d_z^2 on harmonic functions), with the ladders
+//   d_z R_l^m = sqrt((l+m)(l-m)) R_{l-1}^m,
+//   d_+ R_l^m = sqrt((l-m)(l-m-1)) R_{l-1}^{m+1},
+//   d_- R_l^m = -sqrt((l+m)(l+m-1)) R_{l-1}^{m-1}.
+template <class FQ, class FX, class FY, class FZ, class F0, class F1, class F2, class F3, class F4, class F5>
+__device__ double2 apply_moments(int l, int m, FQ Mq, FX Mx, FY My, FZ Mz, F0 Q0, F1 Q1, F2 Q2, F3 Q3, F4 Q4, F5 Q5)
+{
+    double2 acc = Mq(l, m);
+    acc = cadd(acc, cscl(lz(l, m), Mz(l - 1, m)));
+    acc = cadd(acc, cscl(0.5 * lp(l, m), cmi(Mx(l - 1, m + 1), My(l - 1, m + 1))));
+    acc = cadd(acc, cscl(0.5 * lmn(l, m), cpi(Mx(l - 1, m - 1), My(l - 1, m - 1))));
+    if (l >= 2) {
+        double2 d01 = cadd(Q0(l - 2, m + 2), cscl(-1.0, Q1(l - 2, m + 2)));
+        acc = cadd(acc, cscl(0.25 * lp(l, m) * lp(l - 1, m + 1), cmi(d01, Q3(l - 2, m + 2))));
+        d01 = cadd(Q0(l - 2, m - 2), cscl(-1.0, Q1(l - 2, m - 2)));
+        acc = cadd(acc, cscl(0.25 * lmn(l, m) * lmn(l - 1, m - 1), cpi(d01, Q3(l - 2, m - 2))));
+        double2 cz = cadd(Q2(l - 2, m), cscl(-0.5, cadd(Q0(l - 2, m), Q1(l - 2, m))));
+        acc = cadd(acc, cscl(lz(l, m) * lz(l - 1, m), cz));
+        acc = cadd(acc, cscl(0.5 * lz(l, m) * lp(l - 1, m), cmi(Q4(l - 2, m + 1), Q5(l - 2, m + 1))));
+        acc = cadd(acc, cscl(0.5 * lz(l, m) * lmn(l - 1, m), cpi(Q4(l - 2, m - 1), Q5(l - 2, m - 1))));
     }
     return acc;
 }
 
-
-// P2M of B, C1, C2 from the quadrature nodes of a leaf's (mirrored)
-// triangles (l3dtriaformmp_mindlinb_trunc / _mindlinc_trunc,
-// src/mindlinformmptri.f:305, :449 with formmp_mindlinb_trunc
-// src/mindlinbformmp.f and formmp_mindlinc_trunc src/mindlincformmp.f):
-// nodes of the order-nqtri rule (order 6 from iprec 1 up, 4 at iprec 0,
-// 2 below, src/lfmm3dtriamindlin_iter_1.f:679-693) with charge and dipole
-// densities times weight * 2 * area; B: dipole (strength box side, vector sigma_sl) and a
-// quadrupole of the double layer; C1: dipole and charge of sigma_sl,
-// quadrupole and dipole of the double layer; C2: dipole and quadrupole.
-__global__ __launch_bounds__(MW) void k_mind_p2m(MindArgs a, const int *__restrict__ boxes)
+// Pass 2: one thread per (box, coefficient): B, C1, C2 from the moments,
+// conjugated and scaled by side^-l, added to the box's expansions.
+__global__ void k_mind_p2m_combine(MindArgs a, const int *__restrict__ boxes, int nbx, const double2 *__restrict__ G)
 {
-    __shared__ double2 H[MH];
-    int b = boxes[blockIdx.x];
-    int p = a.p;
+    int p = a.p, nown = (p + 1) * (p + 2) / 2;
+    size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (size_t)nbx * nown) return;
+    int i = (int)(gid / nown), idx = (int)(gid % nown);
+    int b = boxes[i];
+    int l = 0;
+    while ((l + 1) * (l + 2) / 2 <= idx) l++;
+    int m = idx - l * (l + 1) / 2;
+    const double2 *Gb = G + (size_t)i * MCOLS * nown;
     double side = a.tsize / (double)(1 << a.level[b]);
-    const double *c = a.centers + 3 * (size_t)b;
-    double al = (a.lam + a.mu) / (a.lam + 2 * a.mu);
-    int nown = (p + 1) * (p + 2) / 2;
-    double2 acc[3][2];
-    for (int e = 0; e < 3; e++)
-        for (int k = 0; k < 2; k++) acc[e][k] = make_double2(0.0, 0.0);
-    int s0 = a.src_start[b], sn = a.src_count[b];
-    int q0 = a.nqtri == 2 ? 0 : a.nqtri == 4 ? 3 : 9, q1 = a.nqtri == 2 ? 3 : a.nqtri == 4 ? 9 : 21;
-    for (int ti = s0; ti < s0 + sn; ti++) {
-        const double *t = a.tri + 9 * (size_t)ti;
-        double ax = t[3] - t[0], ay = t[4] - t[1], az = t[5] - t[2];
-        double bx = t[6] - t[0], by = t[7] - t[1], bz = t[8] - t[2];
-        double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
-        double area = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
-        const double *nv = a.nrm + 3 * (size_t)ti;
-        for (int qn = q0; qn < q1; qn++) {
-            double u = c_mrule[qn][0], v = c_mrule[qn][1];
-            double X[3];
-            for (int d = 0; d < 3; d++) X[d] = t[d] + u * (t[3 + d] - t[d]) + v * (t[6 + d] - t[d]);
-            double w = c_mrule[qn][2] * area * 2;
-            SrcOp S[3];
-            for (int e = 0; e < 3; e++) {
-                S[e].q = 0;
-                for (int k = 0; k < 3; k++) S[e].d[k] = 0;
-                for (int k = 0; k < 6; k++) S[e].Q[k] = 0;
-            }
-            double y3 = X[2];
-            if (a.ifsl) {
-                double sg[3] = {a.ssl[3 * (size_t)ti] * w, a.ssl[3 * (size_t)ti + 1] * w, a.ssl[3 * (size_t)ti + 2] * w};
-                double s0v[3] = {sg[0], sg[1], -sg[2]};
-                for (int k = 0; k < 3; k++) {
-                    S[0].d[k] += side * sg[k];
-                    S[1].d[k] += -al * y3 * s0v[k];
-                    S[2].d[k] += -(2 - al) * s0v[k];
-                }
-                S[1].q += (2 - al) * s0v[2];
-            }
-            if (a.ifdl) {
-                double bv[3] = {a.sdl[3 * (size_t)ti] * w, a.sdl[3 * (size_t)ti + 1] * w, a.sdl[3 * (size_t)ti + 2] * w};
-                double q1 = bv[0] * nv[0], q2 = bv[1] * nv[1], q3 = bv[2] * nv[2];
-                double q4 = bv[0] * nv[1] + bv[1] * nv[0];
-                double q5 = bv[0] * nv[2] + bv[2] * nv[0], q6 = bv[1] * nv[2] + bv[2] * nv[1];
-                double rf = 2.0 * a.mu * side, rr = (q1 + q2 + q3) * side;
-                S[0].Q[0] += q1 * rf;
-                S[0].Q[1] += q2 * rf;
-                S[0].Q[2] += -q3 * rf - a.lam * rr * 2.0;
-                S[0].Q[3] += q4 * rf;
-                double r1 = 2.0 * al * y3 * a.mu;
-                S[1].Q[0] += -r1 * q1;
-                S[1].Q[1] += -r1 * q2;
-                S[1].Q[2] += -r1 * q3;
-                S[1].Q[3] += -r1 * q4;
-                S[1].Q[4] += r1 * q5;
-                S[1].Q[5] += r1 * q6;
-                double r2 = (2.0 - 2.0 * al) * a.mu;
-                S[1].d[0] += -r2 * q5;
-                S[1].d[1] += -r2 * q6;
-                S[1].d[2] += r2 * (q3 + q3) - a.lam * 2.0 * (al - 1) * (q1 + q2 + q3);
-                double r3 = 2.0 * (2.0 - al) * a.mu;
-                S[2].Q[0] += -r3 * q1;
-                S[2].Q[1] += -r3 * q2;
-                S[2].Q[2] += -r3 * q3;
-                S[2].Q[3] += -r3 * q4;
-                S[2].Q[4] += r3 * q5;
-                S[2].Q[5] += r3 * q6;
-            }
-            __syncthreads();
-            harm_regular(p, X[0] - c[0], X[1] - c[1], X[2] - c[2], H);
-            __syncthreads();
-            for (int k = 0; k < 2; k++) {
-                int idx = threadIdx.x + k * MW;
-                if (idx >= nown) break;
-                int l = 0;
-                while ((l + 1) * (l + 2) / 2 <= idx) l++;
-                int m = idx - l * (l + 1) / 2;
-                for (int e = 0; e < 3; e++) {
-                    double2 v = apply_op(S[e], H, l, m);
-                    acc[e][k].x += v.x;
-                    acc[e][k].y -= v.y;          // the multipole takes conj(D R)
-                }
-            }
-        }
-    }
-    for (int k = 0; k < 2; k++) {
-        int idx = threadIdx.x + k * MW;
-        if (idx >= nown) break;
-        int l = 0;
-        while ((l + 1) * (l + 2) / 2 <= idx) l++;
-        double sc = pow(side, -(double)l);
-        for (int e = 0; e < 3; e++) {
-            double *o = a.mp + ((size_t)b * 4 + e) * a.nc + 2 * idx;
-            o[0] += acc[e][k].x * sc;
-            o[1] += acc[e][k].y * sc;
-        }
+    double al = (a.lam + a.mu) / (a.lam + 2 * a.mu), lam = a.lam, mu = a.mu;
+    auto zero = [](int, int) { return make_double2(0.0, 0.0); };
+    auto lin = [&](double w0, int c0) { return [=](int ll, int mm) { return cscl(w0, gmom(Gb, nown, c0, ll, mm)); }; };
+    // B: dipole side * sigma_sl; quadrupole of the double layer
+    double rf = 2.0 * mu * side;
+    auto bq2 = [=](int ll, int mm) {
+        double2 t = cadd(cadd(gmom(Gb, nown, 6, ll, mm), gmom(Gb, nown, 7, ll, mm)), gmom(Gb, nown, 8, ll, mm));
+        return cadd(cscl(-rf, gmom(Gb, nown, 8, ll, mm)), cscl(-2.0 * lam * side, t));
+    };
+    double2 vb = apply_moments(l, m, zero, lin(side, 0), lin(side, 1), lin(side, 2), lin(rf, 6), lin(rf, 7), bq2,
+                               lin(rf, 9), zero, zero);
+    // C1: charge -(2-al) s3, dipole -al y3 (s1, s2, -s3) and the double
+    // layer's (-r2 q5, -r2 q6, 2 r2 q3 - 2 lam (al-1) tr q), quadrupole
+    // -2 al mu y3 (q1, q2, q3, q4, -q5, -q6)
+    double r2 = (2.0 - 2.0 * al) * mu, c1q = 2.0 * al * mu;
+    auto c1x = [=](int ll, int mm) {
+        return cadd(cscl(-al, gmom(Gb, nown, 3, ll, mm)), cscl(-r2, gmom(Gb, nown, 10, ll, mm)));
+    };
+    auto c1y = [=](int ll, int mm) {
+        return cadd(cscl(-al, gmom(Gb, nown, 4, ll, mm)), cscl(-r2, gmom(Gb, nown, 11, ll, mm)));
+    };
+    auto c1z = [=](int ll, int mm) {
+        double2 t = cadd(cadd(gmom(Gb, nown, 6, ll, mm), gmom(Gb, nown, 7, ll, mm)), gmom(Gb, nown, 8, ll, mm));
+        return cadd(cadd(cscl(al, gmom(Gb, nown, 5, ll, mm)), cscl(2.0 * r2, gmom(Gb, nown, 8, ll, mm))),
+                    cscl(-2.0 * lam * (al - 1.0), t));
+    };
+    double2 vc1 = apply_moments(l, m, lin(-(2.0 - al), 2), c1x, c1y, c1z, lin(-c1q, 12), lin(-c1q, 13),
+                                lin(-c1q, 14), lin(-c1q, 15), lin(c1q, 16), lin(c1q, 17));
+    // C2: dipole -(2-al) (s1, s2, -s3); quadrupole -r3 (q1, q2, q3, q4, -q5, -q6)
+    double r3 = 2.0 * (2.0 - al) * mu;
+    double2 vc2 = apply_moments(l, m, zero, lin(-(2.0 - al), 0), lin(-(2.0 - al), 1), lin(2.0 - al, 2), lin(-r3, 6),
+                                lin(-r3, 7), lin(-r3, 8), lin(-r3, 9), lin(r3, 10), lin(r3, 11));
+    double sc = pow(side, -(double)l);
+    double2 v[3] = {vb, vc1, vc2};
+    for (int e = 0; e < 3; e++) {
+        double *o = a.mp + ((size_t)b * 4 + e) * a.nc + 2 * idx;
+        o[0] += v[e].x * sc;
+        o[1] -= v[e].y * sc;                 // the multipole takes conj(D R)
     }
 }
 
@@ -660,11 +723,33 @@ __global__ void k_mind_assemble(int n, int nt, const int *__restrict__ izt, cons
 
 // ---- launchers ----
 
+size_t mind_p2m_nch(int p)
+{
+    size_t nown = (size_t)(p + 1) * (p + 2) / 2, per = nown * 16 + (MCOLS + 3) * 8;
+    return std::min<size_t>(32, MIND_P2M_LDS / per);
+}
+
+size_t mind_p2m_scratch_bytes(int p, int nbx)
+{
+    return (size_t)nbx * MCOLS * ((size_t)(p + 1) * (p + 2) / 2) * 16;
+}
+
 hipError_t launch_mind_p2m(const MindDev &d, const int *boxes, int nbx, hipStream_t st)
 {
     if (nbx <= 0) return hipSuccess;
     const MindArgs &a = d.args;
-    hipLaunchKernelGGL(k_mind_p2m, dim3(nbx), dim3(MW), 0, st, a, boxes);
+    int nown = (a.p + 1) * (a.p + 2) / 2;
+    int nch = (int)mind_p2m_nch(a.p);
+    size_t sh = (size_t)nch * nown * 16 + (size_t)nch * (MCOLS + 3) * 8;
+    if (nown <= 512)
+        hipLaunchKernelGGL(k_mind_moments<512>, dim3(nbx), dim3(512), sh, st, a, boxes, nch, (double2 *)d.gmom);
+    else
+        hipLaunchKernelGGL(k_mind_moments<1024>, dim3(nbx), dim3(1024), sh, st, a, boxes, nch, (double2 *)d.gmom);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    size_t nth = (size_t)nbx * nown;
+    hipLaunchKernelGGL(k_mind_p2m_combine, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0, st, a, boxes, nbx,
+                       (const double2 *)d.gmom);
     return hipGetLastError();
 }
 
@@ -724,6 +809,10 @@ void mind_set_lds_limits(size_t proxy, size_t far)
 {
     (void)hipFuncSetAttribute((const void *)k_mind_proxy, hipFuncAttributeMaxDynamicSharedMemorySize, (int)proxy);
     (void)hipFuncSetAttribute((const void *)k_mind_far, hipFuncAttributeMaxDynamicSharedMemorySize, (int)far);
+    (void)hipFuncSetAttribute((const void *)k_mind_moments<512>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)MIND_P2M_LDS);
+    (void)hipFuncSetAttribute((const void *)k_mind_moments<1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)MIND_P2M_LDS);
     (void)hipGetLastError();
 }
 

@@ -194,9 +194,11 @@ int elfmm_gfx950_triatarg_dev(int iprec, double rlam, double rmu, int nparts,
  * [0] tree+plan (host), [1] upload, [2] P2M, [3] M2M, [4] M2L, [5] P2L,
  * [6] L2L, [7] L2T, [8] M2T, [9] near field, [10] assembly, [11] total
  * wall time of the call, [12] octree and lists (part of [0]), [13]
- * interaction plan (part of [0]), [14]-[15] reserved. [0], [12], [13] are
- * host times; [1]-[10] are HIP-event times on the stream each phase runs
- * on (the near field overlaps [2]-[8]). */
+ * interaction plan (part of [0]); after a half-space call [0]-[13] are
+ * those of its anti-image and image near-field pass, [14] the wall time of
+ * its Mindlin far field and [15] the wall time of the whole call. [0],
+ * [12], [13] are host times; [1]-[10] are HIP-event times on the stream
+ * each phase runs on (the near field overlaps [2]-[8]). */
 void elfmm_gfx950_last_timings(double *ms16);
 
 /* Work counts of the last call on this thread: [0] boxes, [1] levels,

@@ -193,6 +193,11 @@ def half_parts(mask=7):
     library().elfmm_gfx950_half_parts(C.c_int(mask))
 
 
+def expansion_order(iprec):
+    """Expansion order the library uses at iprec (-1 outside -2..3)."""
+    return library().elfmm_gfx950_expansion_order(C.c_int(iprec))
+
+
 def iter_release(wsave):
     """Free the device-resident interactions named by wsave. Returns 1 if
     a store was freed."""

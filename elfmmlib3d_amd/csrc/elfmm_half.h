@@ -30,11 +30,6 @@ size_t hdirect_scratch_bytes(int npt, int n);
 hipError_t launch_hdirect(int npt, const double *pts, int n, const double *tri, const double *nrm, const double *ssl,
                           const double *sdl, int ifsl, int ifdl, int ifpt, int ifst, double lam, double mu,
                           void *scratch, double *pt_out, double *st_out, hipStream_t st);
-// Mindlin harmonic-image far field (lfmm3dmindlintriatarg on the mirrored
-// triangles, src/elhfmm3dtria_iter.f:795-831), added to the target outputs
-int mindlin_far_dev(int iprec, double rlam, double rmu, int n, const double *d_tri, const double *d_nrm,
-                    const double *d_src, int ifsl, const double *d_ssl, int ifdl, const double *d_sdl, int nt,
-                    const double *d_tgt, int ifptt, double *d_ptt, int ifstt, double *d_stt, hipStream_t stream);
 hipError_t launch_mirror(int n, const double *in, double *out, hipStream_t st);
 
 // ---- Mindlin harmonic far field (elfmm_mindlin.hip) ----

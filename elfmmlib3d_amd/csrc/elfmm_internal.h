@@ -104,7 +104,11 @@ struct GemmLaunch {
     std::vector<M2LTile> tiles;
     std::vector<int> src, dst;
     std::vector<double> scale;
-    size_t tile_off = 0, off_src = 0, off_dst = 0, off_sc = 0;   // offsets in the device plan buffer
+};
+// offsets of one GemmLaunch's arrays in a call's device plan buffer (kept
+// by the call, not in the plan, which iter calls of one geometry share)
+struct LaunchOff {
+    size_t tiles = 0, src = 0, dst = 0, sc = 0;
 };
 
 // Task lists of one apply, all box ids 0-based in tree order; points are

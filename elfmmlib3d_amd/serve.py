@@ -48,9 +48,16 @@ class Request:
     ifptfrctarg: int = 1
     ifstraintarg: int = 1
     # iter engine: icomp_type 1/2/3/5 with a caller-owned wsave (numpy,
-    # >= 4 words) that names the device-resident near-field store
+    # >= 4 words) that names the device-resident near-field store; without
+    # a wsave, 2 and 5 compute without keeping a store. Requests that share
+    # a wsave run one at a time, in the order they were submitted.
     icomp_type: Optional[int] = None
     wsave: Optional[np.ndarray] = None
+    # Mindlin half space z < 0 (targets only, as the reference's half-space
+    # driver: the free-space iter apply plus elhfmm3dtriatargiter's image
+    # part, summed). An iter half-space request's wsave holds two handles
+    # (>= 8 words: the free-space store, then the image store).
+    half: bool = False
 
 
 @dataclass
@@ -63,15 +70,17 @@ class Result:
     latency_ms: float
     queue_ms: float
     timings: dict = field(default_factory=dict)
+    wsave: Optional[np.ndarray] = None        # the request's wsave (iter requests)
 
     def numpy(self):
         f = lambda t: None if t is None else t.cpu().numpy()
         return f(self.ptfrc), f(self.strain), f(self.ptfrctarg), f(self.straintarg)
 
 
-def estimate_bytes(n, nt, p=26):
+def estimate_bytes(n, nt, p=26, store_words=0):
     """Device memory of one apply's workspace (expansions, quadrature nodes,
-    near-field records, per-point fields, inputs/outputs), a conservative
+    near-field records, per-point fields, inputs/outputs) at expansion order
+    p, plus the iter store it keeps (store_words doubles), a conservative
     estimate used for admission. The L2T/M2T field operands are formed in
     LDS and take no device memory."""
     nboxes = 2 * (n + nt) // 40 + 64
@@ -79,7 +88,7 @@ def estimate_bytes(n, nt, p=26):
     exps = 2 * nboxes * 4 * nc * 8
     pointwise = (n + nt) * (40 + 9 + 3 + 24) * 8 + n * (12 * 20 + 18 + 24) * 8
     io = (n * 27 + nt * 15) * 8
-    return int(exps + pointwise + io) + (64 << 20)
+    return int(exps + pointwise + io + 8 * store_words) + (64 << 20)
 
 
 class Server:
@@ -98,13 +107,20 @@ class Server:
         self._cv = threading.Condition()
         self._q: "queue.Queue" = queue.Queue()
         self._closed = False
+        self._ws_locks = {}                     # id(wsave) -> lock: one request per store at a time
+        self._ws_mu = threading.Lock()
         self._threads = [threading.Thread(target=self._worker, daemon=True) for _ in range(workers)]
         for t in self._threads:
             t.start()
 
     def submit(self, req: Request) -> Future:
         fut: Future = Future()
-        self._q.put((req, fut, time.perf_counter()))
+        lock = None
+        if req.icomp_type is not None and req.wsave is not None:
+            with self._ws_mu:
+                lock = self._ws_locks.setdefault(id(req.wsave), _Turnstile())
+            lock.ticket(fut)
+        self._q.put((req, fut, time.perf_counter(), lock))
         return fut
 
     def close(self):
@@ -134,26 +150,36 @@ class Server:
             item = self._q.get()
             if item is None:
                 return
-            req, fut, t_submit = item
-            n = req.triangle.shape[0]
-            nt = 0 if req.target is None else req.target.shape[0]
-            need = estimate_bytes(n, nt)
-            self._admit(need)
-            t_start = time.perf_counter()
+            req, fut, t_submit, lock = item
+            if lock is not None:
+                lock.wait_turn(fut)
+            need = 0
             try:
-                res = self._run(req, stream)
+                n = req.triangle.shape[0]
+                nt = 0 if req.target is None else req.target.shape[0]
+                ins = self._inputs(req, stream)
+                store_words = 0
+                if req.icomp_type in (2, 5) and req.wsave is not None:
+                    store_words = self._plan_store_words(req, ins, stream)
+                need = estimate_bytes(n, nt, max(self._lib.expansion_order(req.iprec), 0), store_words)
+                self._admit(need)
+                t_start = time.perf_counter()
+                res = self._run_half(req, ins, stream) if req.half else self._run(req, ins, stream)
                 res.queue_ms = (t_start - t_submit) * 1e3
                 res.latency_ms = (time.perf_counter() - t_submit) * 1e3
                 fut.set_result(res)
             except Exception as e:                       # noqa: BLE001 (reported to the caller)
                 fut.set_exception(e)
             finally:
-                self._release(need)
+                if need:
+                    self._release(need)
+                if lock is not None:
+                    lock.done(fut)
 
-    def _run(self, req: Request, stream):
+    def _inputs(self, req: Request, stream):
+        """The request's arrays on the device and its zeroed outputs."""
         torch = self._torch
         dev = self.device
-        lib = self._lib.library()
 
         def dt(a, shape):
             if a is None:
@@ -170,22 +196,126 @@ class Server:
             ssl, sdl = dt(req.sigma_sl, (n, 3)), dt(req.sigma_dl, (n, 3))
             tgt = dt(req.target, (nt, 3)) if nt else None
             z = lambda shape, flag: torch.zeros(shape, dtype=torch.float64, device=dev) if flag else None
-            pt, st = z((n, 3), req.ifptfrc), z((n, 3, 3), req.ifstrain)
+            pt, st = z((n, 3), req.ifptfrc and not req.half), z((n, 3, 3), req.ifstrain and not req.half)
             ptt, stt = z((nt, 3), req.ifptfrctarg and nt), z((nt, 3, 3), req.ifstraintarg and nt)
             stream.synchronize()
+        return tri, nrm, src, ssl, sdl, tgt, pt, st, ptt, stt
+
+    def _args(self, req, ins):
+        tri, nrm, src, ssl, sdl, tgt, pt, st, ptt, stt = ins
+        n = req.triangle.shape[0]
+        nt = 0 if req.target is None else req.target.shape[0]
         ptr = lambda t: C.c_void_p(0 if t is None else t.data_ptr())
-        s = C.c_void_p(stream.cuda_stream)
-        args = [C.c_int(req.iprec), C.c_double(req.rlam), C.c_double(req.rmu), C.c_int(n), ptr(tri), ptr(nrm),
+        return [C.c_int(req.iprec), C.c_double(req.rlam), C.c_double(req.rmu), C.c_int(n), ptr(tri), ptr(nrm),
                 ptr(src), C.c_int(ssl is not None), ptr(ssl), C.c_int(sdl is not None), ptr(sdl),
                 C.c_int(pt is not None), ptr(pt), C.c_int(st is not None), ptr(st), C.c_int(nt), ptr(tgt),
                 C.c_int(ptt is not None), ptr(ptt), C.c_int(stt is not None), ptr(stt)]
+
+    def _plan_store_words(self, req, ins, stream):
+        """Words of the store(s) an icomp 2/5 request keeps (icomp-4 plans)."""
+        lib = self._lib.library()
+        ws = np.zeros(1)
+        lused = C.c_int(0)
+        s = C.c_void_p(stream.cuda_stream)
+        if not req.half:
+            ier = lib.elfmm_gfx950_triatargiter_dev(*self._args(req, ins), C.c_int(4), ws.ctypes.data_as(C.c_void_p),
+                                                    C.c_int(1), C.byref(lused), s)
+            return int(self._lib.last_stats()["iter_store_words"]) if ier == 0 else 0
+        tri, nrm, src, ssl, sdl, tgt, pt, st, ptt, stt = ins
+        n, nt = req.triangle.shape[0], tgt.shape[0]
+        ptr = lambda t: C.c_void_p(0 if t is None else t.data_ptr())
+        common = [C.c_int(req.iprec), C.c_double(req.rlam), C.c_double(req.rmu), C.c_int(n), ptr(tri), ptr(nrm),
+                  ptr(src), C.c_int(ssl is not None), ptr(ssl), C.c_int(sdl is not None), ptr(sdl)]
+        words = 0
+        ier = lib.elfmm_gfx950_triatargiter_dev(
+            *common, C.c_int(0), ptr(None), C.c_int(0), ptr(None), C.c_int(nt), ptr(tgt), C.c_int(ptt is not None),
+            ptr(ptt), C.c_int(stt is not None), ptr(stt), C.c_int(4), ws.ctypes.data_as(C.c_void_p), C.c_int(1),
+            C.byref(lused), s)
+        words += int(self._lib.last_stats()["iter_store_words"]) if ier == 0 else 0
+        ier = lib.elfmm_gfx950_htriatargiter_dev(
+            *common, C.c_int(nt), ptr(tgt), C.c_int(ptt is not None), ptr(ptt), C.c_int(stt is not None), ptr(stt),
+            C.c_int(4), ws.ctypes.data_as(C.c_void_p), C.c_int(1), C.byref(lused), s)
+        words += int(self._lib.last_stats()["iter_store_words"]) if ier == 0 else 0
+        return words
+
+    def _run(self, req: Request, ins, stream):
+        lib = self._lib.library()
+        tri, nrm, src, ssl, sdl, tgt, pt, st, ptt, stt = ins
+        args = self._args(req, ins)
+        s = C.c_void_p(stream.cuda_stream)
+        ws = None
         if req.icomp_type is None:
             ier = lib.elfmm_gfx950_triatarg_dev(*args, s)
         else:
-            ws = req.wsave if req.wsave is not None else np.zeros(4)
+            # without a wsave there is no room for a handle: no store is kept
+            ws = req.wsave if req.wsave is not None else np.zeros(1)
             lused = C.c_int(0)
             ier = lib.elfmm_gfx950_triatargiter_dev(*args, C.c_int(req.icomp_type),
                                                     ws.ctypes.data_as(C.c_void_p), C.c_int(ws.size),
                                                     C.byref(lused), s)
         timings = self._lib.last_timings()           # same host thread: this request's phases
-        return Result(ier, pt, st, ptt, stt, 0.0, 0.0, timings)
+        return Result(ier, pt, st, ptt, stt, 0.0, 0.0, timings, req.wsave)
+
+    def _run_half(self, req: Request, ins, stream):
+        """Half-space request: free-space iter apply + image part at the
+        targets (examples/elhfmm3dtria_self_iter_dr.f:219-373 sums them)."""
+        torch = self._torch
+        lib = self._lib.library()
+        tri, nrm, src, ssl, sdl, tgt, pt, st, ptt, stt = ins
+        n = req.triangle.shape[0]
+        nt = 0 if req.target is None else req.target.shape[0]
+        ptr = lambda t: C.c_void_p(0 if t is None else t.data_ptr())
+        s = C.c_void_p(stream.cuda_stream)
+        icomp = 1 if req.icomp_type is None else req.icomp_type
+        if req.wsave is not None and req.wsave.size >= 8:
+            wa, wb = req.wsave[0:4], req.wsave[4:8]
+        else:
+            wa, wb = np.zeros(1), np.zeros(1)
+        lused = C.c_int(0)
+        common = [C.c_int(req.iprec), C.c_double(req.rlam), C.c_double(req.rmu), C.c_int(n), ptr(tri), ptr(nrm),
+                  ptr(src), C.c_int(ssl is not None), ptr(ssl), C.c_int(sdl is not None), ptr(sdl)]
+        ier = lib.elfmm_gfx950_triatargiter_dev(
+            *common, C.c_int(0), ptr(None), C.c_int(0), ptr(None), C.c_int(nt), ptr(tgt),
+            C.c_int(ptt is not None), ptr(ptt), C.c_int(stt is not None), ptr(stt), C.c_int(icomp),
+            wa.ctypes.data_as(C.c_void_p), C.c_int(wa.size), C.byref(lused), s)
+        ta = self._lib.last_timings()
+        if ier not in (0, 64):
+            return Result(ier, None, None, ptt, stt, 0.0, 0.0, ta, req.wsave)
+        with torch.cuda.stream(stream):
+            pth = torch.zeros_like(ptt) if ptt is not None else None
+            sth = torch.zeros_like(stt) if stt is not None else None
+        ierh = lib.elfmm_gfx950_htriatargiter_dev(
+            *common, C.c_int(nt), ptr(tgt), C.c_int(pth is not None), ptr(pth), C.c_int(sth is not None), ptr(sth),
+            C.c_int(icomp), wb.ctypes.data_as(C.c_void_p), C.c_int(wb.size), C.byref(lused), s)
+        tb = self._lib.last_timings()
+        with torch.cuda.stream(stream):
+            if ptt is not None:
+                ptt += pth
+            if stt is not None:
+                stt += sth
+            stream.synchronize()
+        timings = {"arrival_" + k: v for k, v in ta.items()}
+        timings.update({"image_" + k: v for k, v in tb.items()})
+        return Result(ierh if ierh not in (0,) else ier, None, None, ptt, stt, 0.0, 0.0, timings, req.wsave)
+
+
+class _Turnstile:
+    """Runs the requests that share one wsave one at a time, in submission
+    order."""
+
+    def __init__(self):
+        self._cv = threading.Condition()
+        self._order = []
+
+    def ticket(self, key):
+        with self._cv:
+            self._order.append(key)
+
+    def wait_turn(self, key):
+        with self._cv:
+            self._cv.wait_for(lambda: self._order and self._order[0] is key)
+
+    def done(self, key):
+        with self._cv:
+            self._order.remove(key)
+            self._cv.notify_all()

@@ -99,8 +99,21 @@ int elfmm_gfx950_triatargiter_dev(int iprec, double rlam, double rmu, int nparts
 
 /* Frees the device-resident interactions named by wsave (returns 1 if
  * freed). No counterpart in the reference, whose wsave is the caller's
- * memory. */
+ * memory.
+ * Sharing a wsave between threads: replays (icomp_type 3) of one store may
+ * run concurrently; a release, or an icomp_type 2/5 call that replaces the
+ * store, while other calls use it frees the old store only when they have
+ * finished. Two icomp_type 2/5 calls on one wsave must not overlap (each
+ * writes its handle into wsave; the store whose handle is lost is never
+ * freed). An icomp_type 3 call replays a store only if the coordinates of
+ * its sources and targets equal those the store was built on; otherwise it
+ * computes directly and returns ier = 64. */
 int elfmm_gfx950_iter_release(const double *wsave, int lwsave);
+
+/* Expansion order the library uses at this iprec (-2..3), or -1. No
+ * counterpart in the reference (l3dterms, src/l3dterms.f:42-85, gives the
+ * same order). */
+int elfmm_gfx950_expansion_order(int iprec);
 
 /* Replaces elhfmm3dtriatargiter (src/elhfmm3dtria_iter.f:46-51): the
  * elastostatic layer potentials of the Mindlin half space z < 0, image part

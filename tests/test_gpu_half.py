@@ -140,3 +140,27 @@ def test_half_space_fmm_vs_direct(gpu_lib):
     assert ier == 0
     _, _, pd, sd = E.elh3dtriadirecttarg(lam, mu, tri, nrm, src, ssl, sdl, tg[sub])
     assert rel(pa + ph, pd) < 0.5e-6 and rel(sa + sh, sd) < 0.5e-5, (rel(pa + ph, pd), rel(sa + sh, sd))
+
+
+def test_half_space_replay_reuses_the_stored_plans(gpu_lib):
+    """A replay (icomp 3) with new densities on a store kept by icomp 2 (its
+    octrees and plans, including the Mindlin far field's, reused) equals an
+    icomp 2 apply with those densities on a fresh store, and a second
+    replay repeats it."""
+    E = gpu_lib
+    g = load_golden("hs_fault_p2")
+    lam, mu, tri, nrm, src, ssl, sdl, tg = inputs(g)
+    ssl2, sdl2 = inputs(g, second=True)[5:7]
+    ws, ws2 = np.zeros(4), np.zeros(4)
+    ier, _, _, _ = E.elhfmm3dtriatargiter(2, lam, mu, tri, nrm, src, ssl, sdl, tg, icomp_type=2, wsave=ws)
+    assert ier == 0
+    ier, _, pt3, st3 = E.elhfmm3dtriatargiter(2, lam, mu, tri, nrm, src, ssl2, sdl2, tg, icomp_type=3, wsave=ws)
+    assert ier == 0
+    assert E.last_timings()["mindlin"] > 0
+    ier, _, pt3b, st3b = E.elhfmm3dtriatargiter(2, lam, mu, tri, nrm, src, ssl2, sdl2, tg, icomp_type=3, wsave=ws)
+    assert ier == 0
+    ier, _, pt2, st2 = E.elhfmm3dtriatargiter(2, lam, mu, tri, nrm, src, ssl2, sdl2, tg, icomp_type=2, wsave=ws2)
+    assert ier == 0
+    assert rel(pt3, pt2) < 1e-13 and rel(st3, st2) < 1e-13
+    assert rel(pt3b, pt3) < 1e-14 and rel(st3b, st3) < 1e-14
+    assert E.iter_release(ws) == 1 and E.iter_release(ws2) == 1

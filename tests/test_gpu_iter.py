@@ -156,3 +156,55 @@ def test_iter_replay_every_store_layout(gpu_lib, layers, fields):
         if on:
             assert rel(a, b) < 1e-12
     assert gpu_lib.iter_release(wsave) == 1
+
+
+def test_iter_replay_on_changed_coordinates_computes_directly(gpu_lib):
+    """A replay whose sources differ from those the store was built on (one
+    centroid moved) computes directly (ier 64) instead of reusing the
+    store's octree and interactions."""
+    g = load_golden("iter_c1_p1")
+    wsave = np.zeros(4)
+    out2 = run(gpu_lib, g, 2, wsave)
+    assert out2[0] == 0
+    src = g["source"].copy()
+    src[7, 0] += 1e-9
+    moved = dict(g)
+    moved["source"] = src
+    out3 = run(gpu_lib, moved, 3, wsave)
+    assert out3[0] == 64
+    out1 = run(gpu_lib, moved, 1)
+    for a, b in zip(out3[2:], out1[2:]):
+        assert rel(a, b) < 1e-13
+    assert gpu_lib.iter_release(wsave) == 1
+
+
+def test_iter_concurrent_replays_of_one_store(gpu_lib):
+    """Replays of one store from several host threads at once equal the
+    same replays made one after another."""
+    import threading
+    g = load_golden("iter_c1_p1")
+    wsave = np.zeros(4)
+    assert run(gpu_lib, g, 2, wsave)[0] == 0
+    n = g["triangle"].shape[0]
+    rng = np.random.default_rng(4)
+    dens = [(rng.uniform(-1, 1, (n, 3)), rng.uniform(-1, 1, (n, 3))) for _ in range(4)]
+
+    def replay(k, out):
+        d = dict(g)
+        d["sigma_sl"], d["sigma_dl"] = dens[k]
+        out[k] = run(gpu_lib, d, 3, wsave)
+
+    alone = [None] * 4
+    for k in range(4):
+        replay(k, alone)
+    together = [None] * 4
+    th = [threading.Thread(target=replay, args=(k, together)) for k in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for a, b in zip(alone, together):
+        assert a[0] == 0 and b[0] == 0
+        for x, y in zip(a[2:], b[2:]):
+            assert rel(x, y) < 1e-13
+    assert gpu_lib.iter_release(wsave) == 1

@@ -47,3 +47,47 @@ def test_results_same_alone_and_in_the_mix(gpu_lib):
     g = load_golden("fmm_c1")
     pt, st, ptt, stt = alone[1].numpy()
     assert rel(pt, g["ptfrc"]) < 1e-10 and rel(stt, g["straintarg"]) < 1e-10
+
+
+def test_iter_and_half_space_requests(gpu_lib):
+    """Iter requests on one wsave (a store, then replays with new densities
+    submitted together: they run one at a time) and half-space requests,
+    beside plain requests; each equals the same call made alone, and the
+    store is released afterwards. An icomp-2 request without a wsave keeps
+    no store."""
+    from elfmmlib3d_amd.serve import Request, Server
+    E = gpu_lib
+    g = load_golden("iter_c1_p1")
+    lam, mu = float(g["lam"]), float(g["mu"])
+    base = (int(g["iprec"]), lam, mu, g["triangle"], g["trinorm"], g["source"])
+    rng = np.random.default_rng(8)
+    n = g["triangle"].shape[0]
+    dens = [(rng.uniform(-1, 1, (n, 3)), rng.uniform(-1, 1, (n, 3))) for _ in range(3)]
+    ws = np.zeros(4)
+    h = load_golden("hs_fault_p2")
+    hs = (int(h["iprec"]), float(h["lam"]), float(h["mu"]), h["triangle"], h["trinorm"], h["source"],
+          h["sigma_sl"], h["sigma_dl"], h["target"])
+    wh = np.zeros(8)
+    srv = Server(workers=3)
+    futs = [srv.submit(Request(*base, g["sigma_sl"], g["sigma_dl"], g["target"], icomp_type=2, wsave=ws))]
+    futs += [srv.submit(Request(*base, a, b, g["target"], icomp_type=3, wsave=ws)) for a, b in dens]
+    futs += [srv.submit(r) for r in requests()[:2]]
+    futs.append(srv.submit(Request(*base, g["sigma_sl"], g["sigma_dl"], g["target"], icomp_type=2)))
+    futs.append(srv.submit(Request(*hs[:6], hs[6], hs[7], hs[8], icomp_type=2, wsave=wh, half=True)))
+    futs.append(srv.submit(Request(*hs[:6], hs[6], hs[7], hs[8], half=True)))
+    res = [f.result() for f in futs]
+    srv.close()
+    assert all(r.ier == 0 for r in res), [r.ier for r in res]
+    for r, (a, b) in zip(res[1:4], dens):
+        _, _, pt, st, ptt, stt = E.elfmm3dtriatargiter(*base, a, b, g["target"], icomp_type=1)
+        for x, y in zip(r.numpy(), (pt, st, ptt, stt)):
+            assert rel(x, y) < 1e-12
+    assert E.iter_release(ws) == 1
+    assert res[6].wsave is None
+    # half space: free-space iter apply + image part at the targets
+    _, _, _, _, pa, sa = E.elfmm3dtriatargiter(*hs[:8], hs[8], ifptfrc=0, ifstrain=0, icomp_type=1)
+    for r, want in ((res[7], (h["anti_pt"] + h["mind_pt"] + h["near2_pt"], h["anti_st"] + h["mind_st"] +
+                                h["near2_st"])), (res[8], (h["full_pt"], h["full_st"]))):
+        _, _, ptt, stt = r.numpy()
+        assert rel(ptt - pa, want[0]) < 1e-10 and rel(stt - sa, want[1]) < 1e-10
+    assert E.iter_release(wh[0:4]) == 1 and E.iter_release(wh[4:8]) == 1

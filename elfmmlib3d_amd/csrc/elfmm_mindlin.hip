@@ -87,35 +87,6 @@ __device__ __forceinline__ double lmn(int l, int m)
     return v > 0 ? -sqrt(v) : 0.0;
 }
 
-// Column m (l = m..L) of the regular harmonics of one point into H[pk(l, m)].
-__device__ void harm_column(int L, int m, double x, double y, double z, double2 *H)
-{
-    double r2 = x * x + y * y + z * z;
-    double dr = 1.0, di = 0.0;
-    for (int k = 1; k <= m; k++) {
-        double f = sqrt((2.0 * k - 1) / (2.0 * k));
-        double nr = -(x * dr - y * di) * f, ni = -(x * di + y * dr) * f;
-        dr = nr;
-        di = ni;
-    }
-    double ar = dr, ai = di, br = 0, bi = 0;
-    H[pk(m, m)] = make_double2(ar, ai);
-    if (m + 1 > L) return;
-    double f = sqrt(2.0 * m + 1) * z;
-    br = f * ar;
-    bi = f * ai;
-    H[pk(m + 1, m)] = make_double2(br, bi);
-    for (int l = m + 2; l <= L; l++) {
-        double f1 = (2.0 * l - 1) * z, f2 = sqrt((l - 1.0 + m) * (l - 1.0 - m)) * r2, d = sqrt((l + m) * (double)(l - m));
-        double cr = (f1 * br - f2 * ar) / d, ci = (f1 * bi - f2 * ai) / d;
-        H[pk(l, m)] = make_double2(cr, ci);
-        ar = br;
-        ai = bi;
-        br = cr;
-        bi = ci;
-    }
-}
-
 // P2M of B, C1, C2 (l3dtriaformmp_mindlinb_trunc / _mindlinc_trunc,
 // src/mindlinformmptri.f:305, :449 with formmp_mindlinb_trunc
 // src/mindlinbformmp.f and formmp_mindlinc_trunc src/mindlincformmp.f) from
@@ -133,35 +104,77 @@ __device__ void harm_column(int L, int m, double x, double y, double z, double2 
 
 constexpr int MCOLS = 18;
 
-// Pass 1: one workgroup per box, nch nodes at a time: node columns and
-// harmonics in LDS, then thread task (coefficient, group of 6 columns)
-// accumulates over the nodes. NT >= (p+1)(p+2)/2.
+// Pass 1: one workgroup per box, thread = coefficient (l, m). The columns
+// are per-triangle values times per-node factors: a triangle's moments are
+// its 9 values (sigma_sl, q) times U = sum_nodes w R(x) and U' = sum_nodes
+// w y3 R(x), w = rule weight * 2 * area. Nodes go through LDS nch at a time
+// (their harmonics by column recurrences with the constants tabulated in
+// LDS); each thread folds a finished triangle into its 18 moments.
+// NT >= (p+1)(p+2)/2.
 template <int NT>
 __global__ __launch_bounds__(NT) void k_mind_moments(MindArgs a, const int *__restrict__ boxes, int nch,
                                                      double2 *__restrict__ G)
 {
     extern __shared__ double smem[];
     int p = a.p, nown = (p + 1) * (p + 2) / 2;
-    double2 *Hs = (double2 *)smem;                       // [nch][nown]
-    double *Ss = (double *)(Hs + (size_t)nch * nown);    // [nch][MCOLS]
-    double *Xs = Ss + nch * MCOLS;                       // [nch][3], relative to the centre
+    double2 *Hs = (double2 *)smem;                          // [nch][nown] harmonics
+    double2 *Rc = Hs + (size_t)nch * nown;                  // [nown] (a, b) of R_l = a z R_{l-1} - b r^2 R_{l-2}
+    double *Dg = (double *)(Rc + nown);                     // [p + 1] sqrt((2k-1)/(2k))
+    double *Ws = Dg + (p + 1);                              // [nch][2] w, w y3
+    double *Tv = Ws + 2 * nch;                              // [nch][9] triangle values at the node
+    double *Xs = Tv + 9 * nch;                              // [nch][3] node - centre
+    int *Is = (int *)(Xs + 3 * nch);                        // [nch] triangle of the node (in the box)
+    int tid = threadIdx.x;
+    for (int k = tid; k < nown; k += NT) {
+        int l = 0;
+        while ((l + 1) * (l + 2) / 2 <= k) l++;
+        int m = k - l * (l + 1) / 2;
+        double ca = 0.0, cb = 0.0;
+        if (l == m + 1) {
+            ca = sqrt(2.0 * m + 1);
+        } else if (l >= m + 2) {
+            double d = 1.0 / sqrt((l + m) * (double)(l - m));
+            ca = (2.0 * l - 1) * d;
+            cb = sqrt((l - 1.0 + m) * (l - 1.0 - m)) * d;
+        }
+        Rc[k] = make_double2(ca, cb);
+    }
+    for (int k = tid; k <= p; k += NT) Dg[k] = k ? sqrt((2.0 * k - 1) / (2.0 * k)) : 1.0;
     int b = boxes[blockIdx.x];
     const double *c = a.centers + 3 * (size_t)b;
     int q0 = a.nqtri == 2 ? 0 : a.nqtri == 4 ? 3 : 9, q1 = a.nqtri == 2 ? 3 : a.nqtri == 4 ? 9 : 21;
     int nqn = q1 - q0;
     int s0 = a.src_start[b];
     int K = a.src_count[b] * nqn;
-    int tid = threadIdx.x, ntask = 3 * nown;
-    double2 acc[3][6];
+    double2 g[MCOLS];
 #pragma unroll
-    for (int j = 0; j < 3; j++)
+    for (int j = 0; j < MCOLS; j++) g[j] = make_double2(0.0, 0.0);
+    double2 U = make_double2(0.0, 0.0), U2 = make_double2(0.0, 0.0);
+    double sv[9];
 #pragma unroll
-        for (int k = 0; k < 6; k++) acc[j][k] = make_double2(0.0, 0.0);
+    for (int j = 0; j < 9; j++) sv[j] = 0.0;
+    int tc = -1;
+    auto flush = [&]() {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            g[k].x += sv[k] * U.x;
+            g[k].y += sv[k] * U.y;
+            g[3 + k].x += sv[k] * U2.x;
+            g[3 + k].y += sv[k] * U2.y;
+        }
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            g[6 + j].x += sv[3 + j] * U.x;
+            g[6 + j].y += sv[3 + j] * U.y;
+            g[12 + j].x += sv[3 + j] * U2.x;
+            g[12 + j].y += sv[3 + j] * U2.y;
+        }
+    };
     for (int k0 = 0; k0 < K; k0 += nch) {
         int nk = min(nch, K - k0);
         __syncthreads();
         if (tid < nk) {
-            int ti = s0 + (k0 + tid) / nqn, qn = q0 + (k0 + tid) % nqn;
+            int tl = (k0 + tid) / nqn, qn = q0 + (k0 + tid) % nqn, ti = s0 + tl;
             const double *t = a.tri + 9 * (size_t)ti;
             double ax = t[3] - t[0], ay = t[4] - t[1], az = t[5] - t[2];
             double bx = t[6] - t[0], by = t[7] - t[1], bz = t[8] - t[2];
@@ -170,17 +183,15 @@ __global__ __launch_bounds__(NT) void k_mind_moments(MindArgs a, const int *__re
             double u = c_mrule[qn][0], v = c_mrule[qn][1];
             double X[3];
             for (int d = 0; d < 3; d++) X[d] = t[d] + u * (t[3 + d] - t[d]) + v * (t[6 + d] - t[d]);
-            double w = c_mrule[qn][2] * area * 2, y3 = X[2];
-            double *s = Ss + tid * MCOLS;
-            for (int k = 0; k < 3; k++) {
-                double sg = a.ifsl ? a.ssl[3 * (size_t)ti + k] * w : 0.0;
-                s[k] = sg;
-                s[3 + k] = y3 * sg;
-            }
+            double w = c_mrule[qn][2] * area * 2;
+            Ws[2 * tid] = w;
+            Ws[2 * tid + 1] = w * X[2];
+            double *tv = Tv + 9 * tid;
+            for (int k = 0; k < 3; k++) tv[k] = a.ifsl ? a.ssl[3 * (size_t)ti + k] : 0.0;
             double q[6] = {0, 0, 0, 0, 0, 0};
             if (a.ifdl) {
                 const double *nv = a.nrm + 3 * (size_t)ti;
-                double bv[3] = {a.sdl[3 * (size_t)ti] * w, a.sdl[3 * (size_t)ti + 1] * w, a.sdl[3 * (size_t)ti + 2] * w};
+                const double *bv = a.sdl + 3 * (size_t)ti;
                 q[0] = bv[0] * nv[0];
                 q[1] = bv[1] * nv[1];
                 q[2] = bv[2] * nv[2];
@@ -188,44 +199,67 @@ __global__ __launch_bounds__(NT) void k_mind_moments(MindArgs a, const int *__re
                 q[4] = bv[0] * nv[2] + bv[2] * nv[0];
                 q[5] = bv[1] * nv[2] + bv[2] * nv[1];
             }
-            for (int k = 0; k < 6; k++) {
-                s[6 + k] = q[k];
-                s[12 + k] = y3 * q[k];
-            }
+            for (int k = 0; k < 6; k++) tv[3 + k] = q[k];
             for (int d = 0; d < 3; d++) Xs[3 * tid + d] = X[d] - c[d];
+            Is[tid] = tl;
         }
         __syncthreads();
+        // harmonics of the chunk's nodes, one column (node, m) per task
         for (int task = tid; task < nk * (p + 1); task += NT) {
             int k = task / (p + 1), m = task % (p + 1);
-            harm_column(p, m, Xs[3 * k], Xs[3 * k + 1], Xs[3 * k + 2], Hs + (size_t)k * nown);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            int t = tid + j * NT;
-            if (t < ntask) {
-                int coef = t % nown, g = t / nown;
-                for (int k = 0; k < nk; k++) {
-                    double2 v = Hs[(size_t)k * nown + coef];
-                    const double *s = Ss + k * MCOLS + 6 * g;
-#pragma unroll
-                    for (int jj = 0; jj < 6; jj++) {
-                        acc[j][jj].x += s[jj] * v.x;
-                        acc[j][jj].y += s[jj] * v.y;
-                    }
+            double x = Xs[3 * k], y = Xs[3 * k + 1], z = Xs[3 * k + 2];
+            double2 *H = Hs + (size_t)k * nown;
+            double r2 = x * x + y * y + z * z;
+            double dr = 1.0, di = 0.0;
+            for (int j = 1; j <= m; j++) {
+                double f = Dg[j];
+                double nr = -(x * dr - y * di) * f, ni = -(x * di + y * dr) * f;
+                dr = nr;
+                di = ni;
+            }
+            double ar = dr, ai = di;
+            H[pk(m, m)] = make_double2(ar, ai);
+            if (m + 1 <= p) {
+                double f = Rc[pk(m + 1, m)].x * z;
+                double br = f * ar, bi = f * ai;
+                H[pk(m + 1, m)] = make_double2(br, bi);
+                for (int l = m + 2; l <= p; l++) {
+                    double2 cc = Rc[pk(l, m)];
+                    double f1 = cc.x * z, f2 = cc.y * r2;
+                    double cr = f1 * br - f2 * ar, ci = f1 * bi - f2 * ai;
+                    H[pk(l, m)] = make_double2(cr, ci);
+                    ar = br;
+                    ai = bi;
+                    br = cr;
+                    bi = ci;
                 }
             }
         }
-    }
+        __syncthreads();
+        if (tid < nown) {
+            for (int k = 0; k < nk; k++) {
+                int t = Is[k];
+                if (t != tc) {
+                    if (tc >= 0) flush();
+                    tc = t;
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-        int t = tid + j * NT;
-        if (t < ntask) {
-            int coef = t % nown, g = t / nown;
-#pragma unroll
-            for (int jj = 0; jj < 6; jj++)
-                G[((size_t)blockIdx.x * MCOLS + 6 * g + jj) * nown + coef] = acc[j][jj];
+                    for (int j = 0; j < 9; j++) sv[j] = Tv[9 * k + j];
+                    U = make_double2(0.0, 0.0);
+                    U2 = make_double2(0.0, 0.0);
+                }
+                double2 v = Hs[(size_t)k * nown + tid];
+                double w = Ws[2 * k], wy = Ws[2 * k + 1];
+                U.x += w * v.x;
+                U.y += w * v.y;
+                U2.x += wy * v.x;
+                U2.y += wy * v.y;
+            }
         }
+    }
+    if (tid < nown) {
+        if (tc >= 0) flush();
+#pragma unroll
+        for (int j = 0; j < MCOLS; j++) G[((size_t)blockIdx.x * MCOLS + j) * nown + tid] = g[j];
     }
 }
 
@@ -723,10 +757,19 @@ __global__ void k_mind_assemble(int n, int nt, const int *__restrict__ izt, cons
 
 // ---- launchers ----
 
+// LDS of the moment pass: nch nodes of (harmonics, 2 weights, 9 triangle
+// values, 3 coordinates, triangle index) and the recurrence tables
+size_t mind_p2m_lds(int p, int nch)
+{
+    size_t nown = (size_t)(p + 1) * (p + 2) / 2;
+    return (size_t)nch * (nown * 16 + (2 + 9 + 3) * 8 + 4) + nown * 16 + (size_t)(p + 1) * 8;
+}
+
 size_t mind_p2m_nch(int p)
 {
-    size_t nown = (size_t)(p + 1) * (p + 2) / 2, per = nown * 16 + (MCOLS + 3) * 8;
-    return std::min<size_t>(32, MIND_P2M_LDS / per);
+    size_t nown = (size_t)(p + 1) * (p + 2) / 2, per = nown * 16 + (2 + 9 + 3) * 8 + 4;
+    size_t fixed = nown * 16 + (size_t)(p + 1) * 8;
+    return std::min<size_t>(32, (MIND_P2M_LDS - fixed) / per);
 }
 
 size_t mind_p2m_scratch_bytes(int p, int nbx)
@@ -740,7 +783,7 @@ hipError_t launch_mind_p2m(const MindDev &d, const int *boxes, int nbx, hipStrea
     const MindArgs &a = d.args;
     int nown = (a.p + 1) * (a.p + 2) / 2;
     int nch = (int)mind_p2m_nch(a.p);
-    size_t sh = (size_t)nch * nown * 16 + (size_t)nch * (MCOLS + 3) * 8;
+    size_t sh = mind_p2m_lds(a.p, nch);
     if (nown <= 512)
         hipLaunchKernelGGL(k_mind_moments<512>, dim3(nbx), dim3(512), sh, st, a, boxes, nch, (double2 *)d.gmom);
     else

@@ -16,6 +16,7 @@
 namespace elfmm {
 
 constexpr int MW = 256;                      // threads per workgroup
+constexpr int MPX = 1024;                    // threads per workgroup of the proxy projection
 constexpr int MP_MAX = 39;                   // largest supported order (iprec 3)
 constexpr int MH = (MP_MAX + 1) * (MP_MAX + 2) / 2;
 constexpr int MPW_FOUR = 544;                // Fourier modes of the largest plane-wave table (525)
@@ -387,9 +388,14 @@ __global__ __launch_bounds__(MW) void k_mind_bmod(MindArgs a, MindPw pw, const i
                                                   const int *__restrict__ slot, const double *__restrict__ ycst,
                                                   int nq, double *__restrict__ pwo, double *__restrict__ skel)
 {
+    constexpr int NBUF = 3 * MPW_FOUR > 2 * MH ? 3 * MPW_FOUR : 2 * MH;
     __shared__ double2 H[MH];
-    __shared__ double2 F[3][MPW_FOUR];
+    __shared__ double2 buf[NBUF];                         // Fourier modes, later the rings' harmonics
     __shared__ double2 ch[2][MP_MAX + 1];
+    __shared__ double2 S[2][MP_MAX + 1];                 // per ring: sum_j q_j e^{-i m phi_j}
+    __shared__ double isf[2 * MP_MAX + 2];                // 1 / sqrt(k!)
+    double2(*F)[MPW_FOUR] = (double2(*)[MPW_FOUR])buf;   // [3][MPW_FOUR]
+    double2(*A)[MH] = (double2(*)[MH])buf;               // [2][MH]: per ring, R_l^m at phi = 0
     int b = boxes[blockIdx.x];
     int p = a.p, nown = (p + 1) * (p + 2) / 2;
     double side = a.tsize / (double)(1 << a.level[b]);
@@ -397,6 +403,15 @@ __global__ __launch_bounds__(MW) void k_mind_bmod(MindArgs a, MindPw pw, const i
     double al = (a.lam + a.mu) / (a.lam + 2 * a.mu);
     double *Mb = a.mp + (size_t)b * 4 * a.nc;
     double *po = pwo + (size_t)slot[b] * 3 * 2 * pw.nexptotp;
+    if (threadIdx.x == 0) {
+        double f = 1.0;
+        isf[0] = 1.0;
+        for (int k = 1; k <= 2 * p + 1; k++) {
+            f *= sqrt((double)k);
+            isf[k] = 1.0 / f;
+        }
+    }
+    __syncthreads();
     // Fourier modes of the three expansions: f_{nl,m} = (-i)^m sum_n (-lam)^n / sqrt((n-m)!(n+m)!) M_n^m
     for (int t = threadIdx.x; t < 3 * pw.nexptot; t += MW) {
         int e = t / pw.nexptot, f = t % pw.nexptot;
@@ -404,20 +419,15 @@ __global__ __launch_bounds__(MW) void k_mind_bmod(MindArgs a, MindPw pw, const i
         while (nl + 1 < pw.nlams && pw.foff[nl + 1] <= f) nl++;
         int mth = f - pw.foff[nl];
         double lam = pw.rlams[nl];
-        double s1r = 0, s1i = 0, s2r = 0, s2i = 0;
+        double sr = 0, si = 0;
         const double *M = Mb + (size_t)e * a.nc;
+        double cc = pow(-lam, (double)mth) * isf[2 * mth];
         for (int n = mth; n <= p; n++) {
-            double cc = pow(-lam, (double)n) / (sqrt_fact(n - mth) * sqrt_fact(n + mth));
             const double *Mm = M + 2 * pk(n, mth);
-            if (((n - mth) & 1) == 0) {
-                s1r += cc * Mm[0];
-                s1i += cc * Mm[1];
-            } else {
-                s2r += cc * Mm[0];
-                s2i += cc * Mm[1];
-            }
+            sr += cc * Mm[0];
+            si += cc * Mm[1];
+            cc *= -lam / sqrt((n + 1.0 - mth) * (n + 1.0 + mth));
         }
-        double sr = s1r + s2r, si = s1i + s2i;
         double zr, zi;
         switch (mth & 3) {
         case 0: zr = 1; zi = 0; break;
@@ -428,7 +438,8 @@ __global__ __launch_bounds__(MW) void k_mind_bmod(MindArgs a, MindPw pw, const i
         F[e][f] = make_double2(sr * zr - si * zi, sr * zi + si * zr);
     }
     __syncthreads();
-    // physical samples
+    // physical samples: f_0 + 2 Re-like sums over the modes with the sign
+    // alternation of ftophys, e^{i m alpha} by rotation
     double scb = (1.0 - al) / al * side;
     for (int t = threadIdx.x; t < 3 * pw.nexptotp; t += MW) {
         int e = t / pw.nexptotp, j = t % pw.nexptotp;
@@ -438,12 +449,17 @@ __global__ __launch_bounds__(MW) void k_mind_bmod(MindArgs a, MindPw pw, const i
         const double2 *f = &F[e][pw.foff[nl]];
         double pr = f[0].x, pi = f[0].y;
         double sgn = -1.0;
+        double c1, s1;
+        sincos(alpha, &s1, &c1);
+        double cr = c1, ci = s1;
         for (int mm = 1; mm < nf; mm++) {
-            double cr = cos(mm * alpha), ci = sin(mm * alpha);
             double tr = cr * f[mm].x - ci * f[mm].y, ti = cr * f[mm].y + ci * f[mm].x;
             pr += tr + sgn * tr;
             pi += ti - sgn * ti;
             sgn = -sgn;
+            double nr = cr * c1 - ci * s1;
+            ci = cr * s1 + ci * c1;
+            cr = nr;
         }
         if (e == 0) {
             double z = 1.0 / (pw.rlams[nl] * pw.rlams[nl]);
@@ -478,8 +494,16 @@ __global__ __launch_bounds__(MW) void k_mind_bmod(MindArgs a, MindPw pw, const i
     for (int j = threadIdx.x; j < 2 * nq; j += MW) {
         int ring = j / nq, jj = j % nq;
         double phi = jj * (2 * M_PI / nq);
+        double c1, s1;
+        sincos(phi, &s1, &c1);
+        double cr = c1, ci = s1;
         double s = ch[ring][0].x;
-        for (int k = 1; k <= p; k++) s += 2 * (ch[ring][k].x * cos(k * phi) - ch[ring][k].y * sin(k * phi));
+        for (int k = 1; k <= p; k++) {
+            s += 2 * (ch[ring][k].x * cr - ch[ring][k].y * ci);
+            double nr = cr * c1 - ci * s1;
+            ci = cr * s1 + ci * c1;
+            cr = nr;
+        }
         sk[4 * j] = c[0] + radius * st * cos(phi);
         sk[4 * j + 1] = c[1] + radius * st * sin(phi);
         sk[4 * j + 2] = c[2] + (ring ? -radius * ct : radius * ct);
@@ -487,30 +511,40 @@ __global__ __launch_bounds__(MW) void k_mind_bmod(MindArgs a, MindPw pw, const i
     }
     __threadfence_block();
     __syncthreads();
-    // multipole of the proxy charges
-    double2 acc[2] = {make_double2(0, 0), make_double2(0, 0)};
-    for (int j = 0; j < 2 * nq; j++) {
-        double x = sk[4 * j] - c[0], y = sk[4 * j + 1] - c[1], z = sk[4 * j + 2] - c[2], q = sk[4 * j + 3];
-        __syncthreads();
-        harm_regular(p, x, y, z, H);
-        __syncthreads();
-        for (int k = 0; k < 2; k++) {
-            int idx = threadIdx.x + k * MW;
-            if (idx >= nown) break;
-            double2 r = H[idx];
-            acc[k].x += q * r.x;
-            acc[k].y -= q * r.y;
+    // multipole of the proxy charges: a ring's charges share R_l^m up to
+    // e^{i m phi_j}, so M_l^m = sum_rings R_l^m(ring, phi = 0) S_ring(m)
+    // with S_ring(m) = sum_j q_j e^{-i m phi_j}
+    for (int t = threadIdx.x; t < 2 * (p + 1); t += MW) {
+        int ring = t / (p + 1), m = t % (p + 1);
+        double sr = 0, si = 0;
+        for (int jj = 0; jj < nq; jj++) {
+            double q = sk[4 * (ring * nq + jj) + 3];
+            double sn, cs;
+            sincos(m * (jj * (2 * M_PI / nq)), &sn, &cs);
+            sr += q * cs;
+            si -= q * sn;
         }
+        S[ring][m] = make_double2(sr, si);
     }
-    // remainder of B (scaled proxies' multipole subtracted), dz^-2, scale
+    harm_regular(p, radius * st, 0.0, radius * ct, A[0]);
+    harm_regular(p, radius * st, 0.0, -radius * ct, A[1]);
     __syncthreads();
+    // remainder of B (scaled proxies' multipole subtracted), dz^-2, scale
     for (int k = 0; k < 2; k++) {
         int idx = threadIdx.x + k * MW;
         if (idx >= nown) break;
         int l = 0;
         while ((l + 1) * (l + 2) / 2 <= idx) l++;
+        int m = idx - l * (l + 1) / 2;
+        double2 mp = make_double2(0.0, 0.0);
+        for (int ring = 0; ring < 2; ring++) {
+            double2 r = A[ring][idx], sv = S[ring][m];
+            // conj(R_l^m(x_j)) q_j = conj(A e^{i m phi_j}) q_j
+            mp.x += r.x * sv.x + r.y * sv.y;
+            mp.y += r.x * sv.y - r.y * sv.x;
+        }
         double sc = pow(side, -(double)l);
-        H[idx] = make_double2(Mb[2 * idx] - acc[k].x * sc, Mb[2 * idx + 1] - acc[k].y * sc);
+        H[idx] = make_double2(Mb[2 * idx] - mp.x * sc, Mb[2 * idx + 1] - mp.y * sc);
     }
     __syncthreads();
     for (int k = 0; k < 2; k++) {
@@ -537,7 +571,7 @@ __global__ __launch_bounds__(MW) void k_mind_bmod(MindArgs a, MindPw pw, const i
 // the local expansion of B (l3dformtaintkerbf90sc, src/mindlinformta.f:980).
 // gl: Gauss-Legendre nodes (nquad) then weights (nquad); ytab: ylgndr at the
 // nodes, [nquad][(p+1)^2]; eph: e^{-i m 2 pi (k+1)/nquad}, [m][nquad].
-__global__ __launch_bounds__(MW) void k_mind_proxy(MindArgs a, const int *__restrict__ tboxes,
+__global__ __launch_bounds__(MPX) void k_mind_proxy(MindArgs a, const int *__restrict__ tboxes,
                                                    const int *__restrict__ tstart, const int *__restrict__ titems,
                                                    const int *__restrict__ slot, const double *__restrict__ skel,
                                                    int nq, int nquad, const double *__restrict__ gl,
@@ -561,9 +595,9 @@ __global__ __launch_bounds__(MW) void k_mind_proxy(MindArgs a, const int *__rest
         double sc = (1.0 - al) / al / sside;              // times the source box's scale
         const double *ks = skel + (size_t)slot[s] * 2 * nq * 4;
         __syncthreads();
-        for (int k = threadIdx.x; k < 2 * nq * 4; k += MW) sk[k] = ks[k];
+        for (int k = threadIdx.x; k < 2 * nq * 4; k += MPX) sk[k] = ks[k];
         __syncthreads();
-        for (int g = threadIdx.x; g < nquad * nquad; g += MW) {
+        for (int g = threadIdx.x; g < nquad * nquad; g += MPX) {
             int jj = g / nquad, kk = g % nquad;
             double cth = gl[jj], sth = sqrt(1.0 - cth * cth);
             double ph = 2 * M_PI * (kk + 1) / nquad;
@@ -577,7 +611,7 @@ __global__ __launch_bounds__(MW) void k_mind_proxy(MindArgs a, const int *__rest
             phi[g] = v * sc;
         }
         __syncthreads();
-        for (int g = threadIdx.x; g < nquad * (p + 1); g += MW) {
+        for (int g = threadIdx.x; g < nquad * (p + 1); g += MPX) {
             int jj = g / (p + 1), m = g % (p + 1);
             double sr = 0, si = 0;
             for (int kk = 0; kk < nquad; kk++) {
@@ -589,7 +623,7 @@ __global__ __launch_bounds__(MW) void k_mind_proxy(MindArgs a, const int *__rest
         }
         __syncthreads();
         for (int k = 0; k < 2; k++) {
-            int idx = threadIdx.x + k * MW;
+            int idx = threadIdx.x + k * MPX;
             if (idx >= nown) break;
             int l = 0;
             while ((l + 1) * (l + 2) / 2 <= idx) l++;
@@ -607,7 +641,7 @@ __global__ __launch_bounds__(MW) void k_mind_proxy(MindArgs a, const int *__rest
         }
     }
     for (int k = 0; k < 2; k++) {
-        int idx = threadIdx.x + k * MW;
+        int idx = threadIdx.x + k * MPX;
         if (idx >= nown) break;
         double *o = a.lo + (size_t)t * 4 * a.nc + 2 * idx;
         o[0] += acc[k].x;
@@ -817,7 +851,7 @@ hipError_t launch_mind_proxy(const MindDev &d, const int *tb, const int *ts, con
     if (nbx <= 0) return hipSuccess;
     const MindArgs &a = d.args;
     size_t sh = (size_t)d.nquad * d.nquad * 8 + (size_t)d.nquad * (a.p + 1) * 16 + (size_t)2 * d.nq * 4 * 8;
-    hipLaunchKernelGGL(k_mind_proxy, dim3(nbx), dim3(MW), sh, st, a, tb, ts, ti, d.slot, d.skel, d.nq, d.nquad, d.gl,
+    hipLaunchKernelGGL(k_mind_proxy, dim3(nbx), dim3(MPX), sh, st, a, tb, ts, ti, d.slot, d.skel, d.nq, d.nquad, d.gl,
                        d.ytab, (const double2 *)d.eph);
     return hipGetLastError();
 }

@@ -31,49 +31,239 @@ PEAK_HBM = 8.0e12          # bytes/s
 # strain), counted by tests/csrc/nf_opcount.cpp on the library's pair code:
 # 862 adds, 1441 muls, 31 divs, 25 sqrt/log/atan2
 NEAR_OPS_PER_PAIR = 862 + 1441 + 31 + 25
-# HBM bytes per launch on this workload, from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE, each scaled by what the counter reads for a 512 MiB device copy
-# in the same run (profiles/c2_apply_kernels.md)
-TRAFFIC = {"k_near": 211.5e6, "k_m2l": 8148.0e6}
+CALIB_MIB = 512            # device copy whose counters scale FETCH_SIZE / WRITE_SIZE to bytes
 
 
 def workload(nsplit=64, ntarg=100000, seed=2024):
     from elfmmlib3d_amd import mesh
-    rng = np.random.default_rng(seed)
-    tri = mesh.icosphere(nsplit)
-    nrm = mesh.triangle_normals(tri)
-    src = mesh.centroids(tri)
-    ssl = rng.uniform(-1, 1, (tri.shape[0], 3))
-    sdl = rng.uniform(-1, 1, (tri.shape[0], 3))
-    tg = mesh.cube_targets(ntarg, 2.0, 1.0, rng)
-    return tri, nrm, src, ssl, sdl, tg
+    return mesh.sphere_workload(nsplit, ntarg, seed)
 
 
-def cpu_baseline(seconds_budget=30.0):
-    """The reference's own FMM (oracle/_ref, OpenMP) on a bounded sample of
-    the same workload: icosphere 16x16 per face (5,120 triangles) with 6,250
-    targets in the same cube (same target/triangle ratio as C2), iprec 2."""
+def cpu_baseline():
+    """The reference's own FMM (oracle/_ref: flang -O1 -fopenmp, OpenMP over
+    the host cores this process may use, OMP_STACKSIZE 1024M) on this host:
+    C1 (icosphere 12x12/face, 2,880 triangles + 1,000 shell targets) as a
+    warm-up and its own figure (median of 5 applies), then the full C2
+    workload (81,920 triangles + 100,000 targets), one apply."""
     from oracle import ref
     if not ref.available():
         return None
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    from elfmmlib3d_amd import mesh
+    os.environ.setdefault("OMP_STACKSIZE", "1024M")
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or ncpu
     os.environ["OMP_NUM_THREADS"] = str(threads)
-    tri, nrm, src, ssl, sdl, tg = workload(16, 6250, 7)
-    pts = tri.shape[0] + tg.shape[0]
-    times = []
-    t_end = time.time() + seconds_budget
-    while True:
+    rng = np.random.default_rng(1)
+    tri = mesh.icosphere(12)
+    c1 = (tri, mesh.triangle_normals(tri), mesh.centroids(tri), rng.uniform(-1, 1, (tri.shape[0], 3)),
+          rng.uniform(-1, 1, (tri.shape[0], 3)), mesh.shell_targets(1000, 1.5, 2.0, rng))
+    t1 = []
+    for _ in range(5):
         t0 = time.time()
-        ier, *_ = ref.elfmm3dtriatarg(2, 1.0, 1.0, tri, nrm, src, ssl, sdl, tg)
-        times.append(time.time() - t0)
-        if time.time() > t_end or len(times) >= 3:
-            break
-    t = float(np.median(times))
-    return {"value": pts / t / 1e6, "unit": "M tris+targets/s", "cores": threads,
-            "kind": "reference",
-            "sample": f"reference elfmm3dtriatarg (flang -O1 -fopenmp), icosphere 16x16/face "
-                      f"(5120 tris) + 6250 cube targets, iprec 2, median of {len(times)} applies "
-                      f"({t:.2f} s each)"}
+        ier, *_ = ref.elfmm3dtriatarg(2, 1.0, 1.0, *c1)
+        t1.append(time.time() - t0)
+    c1_t = float(np.median(t1))
+    c2 = workload()
+    t0 = time.time()
+    ier, *_ = ref.elfmm3dtriatarg(2, 1.0, 1.0, *c2)
+    c2_t = time.time() - t0
+    pts = c2[0].shape[0] + c2[5].shape[0]
+    return {"value": pts / c2_t / 1e6, "unit": "M tris+targets/s", "cores": threads, "kind": "reference",
+            "sample": f"reference elfmm3dtriatarg (flang -O1 -fopenmp, OMP_STACKSIZE 1024M) on the full C2 "
+                      f"workload, one apply after 5 C1 applies: {c2_t:.1f} s; {threads} threads of "
+                      f"{os.cpu_count()} CPUs",
+            "c1": {"value": (c1[0].shape[0] + c1[5].shape[0]) / c1_t / 1e6, "s_per_apply": c1_t,
+                   "applies": len(t1)},
+            "c2_s_per_apply": c2_t}
+
+
+def kernel_pass(args):
+    """Child of the traffic measurement: a 512 MiB device copy (counter
+    calibration), then two applies with the near field on the apply's own
+    stream, under rocprofv3 --pmc."""
+    import torch
+    import elfmmlib3d_amd as E
+    os.environ["ELFMM_SERIAL_NEAR"] = "1"
+    dev = torch.device("cuda:0")
+    tri, nrm, src, ssl, sdl, tg = workload(args.nsplit, args.ntarg)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    n, m = tri.shape[0], tg.shape[0]
+    z = lambda *sh: torch.empty(sh, dtype=torch.float64, device=dev)
+    a = torch.ones(CALIB_MIB << 17, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    ins = [T(x) for x in (tri, nrm, src, ssl, sdl, tg)]
+    outs = [z(n, 3), z(n, 3, 3), z(m, 3), z(m, 3, 3)]
+    for _ in range(2):
+        ier = E.elfmm3dtriatarg_dev(args.iprec, 1.0, 1.0, *ins, *outs)
+        assert ier == 0
+    torch.cuda.synchronize()
+
+
+def measure_traffic(args):
+    """HBM bytes per dispatch of every kernel of the apply: two rocprofv3
+    --pmc runs of kernel_pass (FETCH_SIZE, then WRITE_SIZE, KiB per
+    dispatch), each scaled by what the counter reads for the 512 MiB device
+    copy of the same run. Returns ({kernel: bytes}, factors) or (None, why)."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None, "rocprofv3 not found"
+    kib = {}
+    factors = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            out = os.path.join(tmp, counter)
+            cmd = [prof, "--pmc", counter, "-f", "csv", "-d", out, "-o", counter, "--", sys.executable,
+                   os.path.abspath(__file__), "--kernel-pass", "--iprec", str(args.iprec), "--nsplit",
+                   str(args.nsplit), "--ntarg", str(args.ntarg)]
+            env = dict(os.environ, TMPDIR=tmp)
+            try:
+                r = subprocess.run(cmd, cwd=tmp, env=env, capture_output=True, text=True, timeout=300)
+            except subprocess.TimeoutExpired:
+                return None, f"rocprofv3 {counter} pass timed out"
+            files = glob.glob(os.path.join(out, "**", "*counter_collection*.csv"), recursive=True)
+            if r.returncode != 0 and not files:
+                return None, f"rocprofv3 {counter} pass failed ({r.returncode})"
+            per = {}
+            for f in files:
+                with open(f) as fh:
+                    for row in csv.DictReader(fh):
+                        name = row.get("Kernel_Name", "")
+                        if row.get("Counter_Name", counter) != counter:
+                            continue
+                        per.setdefault(name, []).append(float(row["Counter_Value"]))
+            copies = [v for k, vs in per.items() if "copyBuffer" in k for v in vs]
+            if not copies:
+                return None, f"no device copy in the {counter} pass"
+            factors[counter] = (CALIB_MIB << 20) / (max(copies) * 1024.0)
+            for name, vs in per.items():
+                short = name.split("(")[0].split("::")[-1].split("<")[0]
+                kib.setdefault(short, {})[counter] = float(np.mean(vs))
+    traffic = {k: v.get("FETCH_SIZE", 0.0) * 1024 * factors["FETCH_SIZE"] +
+               v.get("WRITE_SIZE", 0.0) * 1024 * factors["WRITE_SIZE"] for k, v in kib.items()
+               if k.startswith("k_")}
+    return traffic, factors
+
+
+def c5_served(args):
+    """Config C5 on one GPU: a Poisson stream of requests at 0.5x, 1x and 2x
+    the measured capacity; 60% half-space fault meshes (8 faults, targets =
+    centroids) and 40% free-space spheres (centroids + ntri/2 cube targets),
+    triangle counts log-uniform in [1k, args.c5_max]; 30% of the requests
+    reuse one of three resident geometries with new densities (iter mode:
+    its near fields stored before the stream, replayed by the request). A
+    pool of geometries is generated beforehand (request = geometry + fresh
+    densities; a non-resident request plans from scratch). Latency = result
+    time - arrival time. Returns the per-rate figures."""
+    import torch
+    from elfmmlib3d_amd import mesh
+    from elfmmlib3d_amd.serve import Request, Server
+    rng = np.random.default_rng(11)
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+    def geometry(kind, ntri, seed):
+        g = np.random.default_rng(seed)
+        if kind == "half":
+            tri = mesh.fault_mesh(8, ntri, g)
+            tg = mesh.centroids(tri)
+        else:
+            k = max(1, int(round(np.sqrt(ntri / 20.0))))
+            tri = mesh.icosphere(k, radius=g.uniform(0.5, 3.0))
+            tg = mesh.cube_targets(max(1, tri.shape[0] // 2), 4.0, 3.1, g)
+        return {"kind": kind, "tri": T(tri), "nrm": T(mesh.triangle_normals(tri)), "src": T(mesh.centroids(tri)),
+                "tg": T(tg), "n": tri.shape[0], "m": tg.shape[0]}
+
+    lo, hi = np.log(1000.0), np.log(float(args.c5_max))
+    pool = [geometry("half" if i % 5 < 3 else "sphere", int(np.exp(rng.uniform(lo, hi))), 100 + i)
+            for i in range(10)]
+    resident = [geometry("half", 20000, 901), geometry("sphere", 40000, 902), geometry("half", 60000, 903)]
+
+    def request(geo, icomp=None, wsave=None):
+        n = geo["n"]
+        dl = torch.from_numpy(rng.uniform(-1, 1, (n, 3))).to(dev)
+        sl = None if geo["kind"] == "half" else torch.from_numpy(rng.uniform(-1, 1, (n, 3))).to(dev)
+        return Request(args.iprec, 1.0, 1.0, geo["tri"], geo["nrm"], geo["src"], sl, dl, geo["tg"],
+                       ifptfrc=0 if geo["kind"] == "half" else 1, ifstrain=0 if geo["kind"] == "half" else 1,
+                       icomp_type=icomp, wsave=wsave, half=geo["kind"] == "half")
+
+    srv = Server(workers=args.workers)
+    wss = [np.zeros(8) for _ in resident]
+    for geo, ws in zip(resident, wss):
+        r = srv.submit(request(geo, 2, ws)).result()
+        assert r.ier == 0, r.ier
+
+    # one sequence of (geometry, resident slot) for every phase, fresh
+    # densities per request
+    seq = []
+    for _ in range(args.c5_requests):
+        if rng.uniform() < 0.3:
+            k = int(rng.integers(len(resident)))
+            seq.append((resident[k], k))
+        else:
+            seq.append((pool[int(rng.integers(len(pool)))], -1))
+
+    def requests():
+        return [(request(geo, 3, wss[k]) if k >= 0 else request(geo), geo) for geo, k in seq]
+
+    def points(geo):
+        return geo["n"] + geo["m"] if geo["kind"] == "sphere" else geo["m"]
+
+    # capacity: the sequence submitted at once (the server keeps `workers` in flight)
+    reqs = requests()
+    t0 = time.perf_counter()
+    futs = [srv.submit(r) for r, _ in reqs]
+    res = [f.result() for f in futs]
+    cap_s = time.perf_counter() - t0
+    assert all(r.ier in (0, 64) for r in res), [r.ier for r in res]
+    capacity = len(reqs) / cap_s
+    out = {"capacity_req_per_s": capacity, "requests_per_rate": args.c5_requests,
+           "pool_triangles": sorted(int(g["n"]) for g in pool),
+           "resident_triangles": [int(g["n"]) for g in resident]}
+    rates = []
+    for mult in (0.5, 1.0, 2.0):
+        rate = capacity * mult
+        reqs = requests()
+        gaps = rng.exponential(1.0 / rate, len(reqs))
+        t_start = time.perf_counter()
+        futs = []
+        t_next = t_start
+        for (r, geo), gap in zip(reqs, gaps):
+            t_next += gap
+            d = t_next - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            futs.append(srv.submit(r))
+        pts = 0
+        for (r, geo), f in zip(reqs, futs):
+            res = f.result()
+            assert res.ier in (0, 64), res.ier
+            pts += points(geo)
+        t_end = time.perf_counter()
+        # latency: submission (at the arrival time) to result, as the server measured it
+        lat_srv = np.array([f.result().latency_ms for f in futs])
+        rates.append({"offered_req_per_s": rate, "x_capacity": mult,
+                      "throughput_M_points_per_s": pts / (t_end - t_start) / 1e6,
+                      "completed_req_per_s": len(reqs) / (t_end - t_start),
+                      "latency_ms": {"p50": float(np.percentile(lat_srv, 50)),
+                                     "p99": float(np.percentile(lat_srv, 99))}})
+    srv.close()
+    from elfmmlib3d_amd import _lib
+    for ws in wss:
+        _lib.iter_release(ws[0:4])
+        _lib.iter_release(ws[4:8])
+    out["rates"] = rates
+    return out
 
 
 def main():
@@ -92,7 +282,16 @@ def main():
                     help="skip the stored near-field replay measurement")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="run one device copy of MIB MiB first (HBM counter calibration under rocprofv3)")
+    ap.add_argument("--no-traffic", dest="traffic", action="store_false",
+                    help="skip the rocprofv3 counter passes that measure each kernel's HBM traffic")
+    ap.add_argument("--kernel-pass", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--no-c5", dest="c5", action="store_false", help="skip the C5 served-stream measurement")
+    ap.add_argument("--c5-requests", type=int, default=30, help="requests per offered rate in C5")
+    ap.add_argument("--c5-max", type=int, default=500000, help="largest triangle count of a C5 request")
     args = ap.parse_args()
+    if args.kernel_pass:
+        kernel_pass(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "the library runs on one GPU per process; --gpus must be 1"}))
         sys.exit(2)
@@ -272,9 +471,13 @@ def main():
     roof = rooflines.get(dom, {"kernel": dom, "bound": "fp64", "achieved": None, "peak": PEAK_FP64 / 1e12,
                                "unit": "TFLOP/s", "frac": None, "traffic": None,
                                "ms_per_launch": kernels[dom]})
-    if TRAFFIC.get(roof["kernel"]) is not None:
-        roof["traffic"] = TRAFFIC[roof["kernel"]]
+    traffic, factors = measure_traffic(args) if args.traffic else (None, "not measured (--no-traffic)")
+    if traffic:
+        for r in list(rooflines.values()) + [roof]:
+            if r.get("kernel") in traffic:
+                r["traffic"] = traffic[r["kernel"]]
 
+    c5 = c5_served(args) if args.c5 else None
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "FMM apply throughput (M tris+targets/s) @1e-6 err; p50/p99 served latency",
@@ -296,6 +499,11 @@ def main():
         "phases_ms": ph, "kernel_phases_ms": kph, "work": stats,
         "roofline": roof, "rooflines": dict(rooflines, **({"iter_replay": replay} if replay else {})),
         "cpu_baseline": cpu,
+        "served_c5": c5,
+        "traffic": ({"bytes_per_dispatch": traffic, "counter_scale": factors,
+                     "method": f"rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, over a child run of two applies; "
+                               f"KiB per dispatch scaled by the counters of a {CALIB_MIB} MiB device copy"}
+                    if traffic else {"error": factors}),
     }
     if cpu:
         line["speedup_vs_cpu_reference"] = value / cpu["value"]

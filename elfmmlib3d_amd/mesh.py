@@ -177,3 +177,18 @@ def fault_mesh(nfaults, ntri_total, rng, box=20.0, top=(0.0, 3.0)):
                 tris.append((p00, p10, p11))
                 tris.append((p00, p11, p01))
     return np.ascontiguousarray(np.array(tris))
+
+
+def sphere_workload(nsplit=64, ntarg=100000, seed=2024):
+    """The sphere workload of the benchmark (C2 at the defaults): icosphere
+    nsplit x nsplit per face, densities uniform in [-1,1]^3, ntarg targets
+    uniform in [-2,2]^3 minus the unit ball. Returns (triangle, trinorm,
+    source, sigma_sl, sigma_dl, target)."""
+    rng = np.random.default_rng(seed)
+    tri = icosphere(nsplit)
+    nrm = triangle_normals(tri)
+    src = centroids(tri)
+    ssl = rng.uniform(-1, 1, (tri.shape[0], 3))
+    sdl = rng.uniform(-1, 1, (tri.shape[0], 3))
+    tg = cube_targets(ntarg, 2.0, 1.0, rng)
+    return tri, nrm, src, ssl, sdl, tg

@@ -180,6 +180,33 @@ def hs_low_case(name, lam=1.1, mu=0.9):
          **out)
 
 
+def c2_case(name, iprecs=(2, 3), nsample=3000, nproj=6):
+    """C2 (icosphere 64x64/face, 81,920 triangles, 100,000 targets, seed
+    2024; mesh.sphere_workload) through the reference FMM at each iprec:
+    outputs at seeded sample indices (sources and targets) and the full
+    outputs' dot products with seeded Gaussian vectors. The inputs are not
+    stored: mesh.sphere_workload regenerates them."""
+    tri, nrm, src, ssl, sdl, tg = mesh.sphere_workload(64, 100000, 2024)
+    n, m = tri.shape[0], tg.shape[0]
+    rng = np.random.default_rng(77)
+    si = np.sort(rng.choice(n, nsample, replace=False))
+    ti = np.sort(rng.choice(m, nsample, replace=False))
+    out = dict(nsplit=64, ntarg=100000, seed=2024, lam=1.0, mu=1.0, src_idx=si, tgt_idx=ti,
+               proj_seed=78, nproj=nproj)
+    for ip in iprecs:
+        ier, pt, st, ptt, stt = ref.elfmm3dtriatarg(ip, 1.0, 1.0, tri, nrm, src, ssl, sdl, tg)
+        assert ier == 0
+        prng = np.random.default_rng(78)
+        for k, a in (("ptfrc", pt), ("strain", st), ("ptfrctarg", ptt), ("straintarg", stt)):
+            v = a.reshape(a.shape[0], -1)
+            out[f"p{ip}_{k}_sample"] = v[si if k in ("ptfrc", "strain") else ti]
+            R = prng.standard_normal((nproj, v.size))
+            out[f"p{ip}_{k}_proj"] = R @ v.ravel()
+            out[f"p{ip}_{k}_norm"] = np.linalg.norm(v)
+        print("c2 iprec", ip, "done", flush=True)
+    save(name, **out)
+
+
 def main(only=None):
     want = lambda name: only is None or name in only
     tri, nrm, src, ssl, sdl, tg = sphere_case(6, 300, 11)
@@ -205,6 +232,8 @@ def main(only=None):
         hs_case("hs_fault_p2", 2)
     if want("hs_fault_p1"):
         hs_case("hs_fault_p1", 1)
+    if only is not None and "c2_ref" in only:
+        c2_case("c2_ref")
     if want("hs_fault_low"):
         hs_low_case("hs_fault_low")
     if want("fmm_fault"):

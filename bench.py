@@ -155,6 +155,90 @@ def measure_traffic(args):
     return traffic, factors
 
 
+def c3_half(args):
+    """Config C3 on one GPU: 8 planar faults in z < 0 (mesh.fault_mesh, seed
+    33) meshed into about args.c3_tri right triangles, slip (double layer)
+    uniform in [-1,1]^3, displacement and strain at the centroids, iprec 2:
+    the reference driver's sequence (icomp 4, 2, then 3 with new slip) for
+    the direct arrival (elfmm3dtriatargiter) and the image part
+    (elhfmm3dtriatargiter). An apply = both replays; value = triangles per
+    second of an apply. The image-part kernels' times come from the
+    library's phase timers of the same calls."""
+    import torch
+    import elfmmlib3d_amd as E
+    from elfmmlib3d_amd import mesh
+    rng = np.random.default_rng(33)
+    tri = mesh.fault_mesh(8, args.c3_tri, rng)
+    nrm, src = mesh.triangle_normals(tri), mesh.centroids(tri)
+    n = tri.shape[0]
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    d_tri, d_nrm, d_src = T(tri), T(nrm), T(src)
+    z = lambda *sh: torch.zeros(sh, dtype=torch.float64, device=dev)
+    pa, sa, ph, sh = z(n, 3), z(n, 3, 3), z(n, 3), z(n, 3, 3)
+    lib = E.library()
+    import ctypes as C
+    ptr = lambda t: C.c_void_p(0 if t is None else t.data_ptr())
+    stream = C.c_void_p(0)
+    wa, wb = np.zeros(4), np.zeros(4)
+    lused = C.c_int(0)
+
+    def arrival(icomp, slip):
+        ier = lib.elfmm_gfx950_triatargiter_dev(
+            C.c_int(args.iprec), C.c_double(1.0), C.c_double(1.0), C.c_int(n), ptr(d_tri), ptr(d_nrm), ptr(d_src),
+            C.c_int(0), ptr(None), C.c_int(1), ptr(slip), C.c_int(0), ptr(None), C.c_int(0), ptr(None), C.c_int(n),
+            ptr(d_src), C.c_int(1), ptr(pa), C.c_int(1), ptr(sa), C.c_int(icomp), wa.ctypes.data_as(C.c_void_p),
+            C.c_int(4), C.byref(lused), stream)
+        return ier, E.last_stats()["iter_store_words"]
+
+    def image(icomp, slip):
+        ier = lib.elfmm_gfx950_htriatargiter_dev(
+            C.c_int(args.iprec), C.c_double(1.0), C.c_double(1.0), C.c_int(n), ptr(d_tri), ptr(d_nrm), ptr(d_src),
+            C.c_int(0), ptr(None), C.c_int(1), ptr(slip), C.c_int(n), ptr(d_src), C.c_int(1), ptr(ph), C.c_int(1),
+            ptr(sh), C.c_int(icomp), wb.ctypes.data_as(C.c_void_p), C.c_int(4), C.byref(lused), stream)
+        return ier, E.last_timings()
+
+    try:
+        slip = T(rng.uniform(-1, 1, (n, 3)))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ier_a, words_a = arrival(2, slip)
+        torch.cuda.synchronize()
+        t_store_a = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ier_b, _ = image(2, slip)
+        words_b = E.last_stats()["iter_store_words"]
+        torch.cuda.synchronize()
+        t_store_b = time.perf_counter() - t0
+        assert ier_a == 0 and ier_b == 0, (ier_a, ier_b)
+        ta, tb, tim = [], [], []
+        for _ in range(args.c3_replays):
+            slip = T(rng.uniform(-1, 1, (n, 3)))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ier, _ = arrival(3, slip)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            ier2, tm = image(3, slip)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            assert ier == 0 and ier2 == 0, (ier, ier2)
+            ta.append(t1 - t0)
+            tb.append(t2 - t1)
+            tim.append(tm)
+        apply_s = float(np.median(np.array(ta) + np.array(tb)))
+        return {"workload": f"C3: 8 faults, {n} triangles, slip U[-1,1]^3, centroids, iprec {args.iprec}",
+                "value": n / apply_s / 1e6, "unit": "M tris/s", "apply_ms": apply_s * 1e3,
+                "arrival_replay_ms": float(np.median(ta)) * 1e3, "image_replay_ms": float(np.median(tb)) * 1e3,
+                "image_mindlin_ms": float(np.median([x["mindlin"] for x in tim])),
+                "store_s": {"arrival": t_store_a, "image": t_store_b},
+                "store_gb": {"arrival": words_a * 8 / 1e9, "image": words_b * 8 / 1e9},
+                "replays": args.c3_replays}
+    finally:
+        E.iter_release(wa)
+        E.iter_release(wb)
+
+
 def c5_served(args):
     """Config C5 on one GPU: a Poisson stream of requests at 0.5x, 1x and 2x
     the measured capacity; 60% half-space fault meshes (8 faults, targets =
@@ -286,11 +370,18 @@ def main():
                     help="skip the rocprofv3 counter passes that measure each kernel's HBM traffic")
     ap.add_argument("--kernel-pass", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-c5", dest="c5", action="store_false", help="skip the C5 served-stream measurement")
+    ap.add_argument("--no-c3", dest="c3", action="store_false", help="skip the C3 half-space measurement")
+    ap.add_argument("--c3-tri", type=int, default=200000, help="triangles of the C3 fault mesh")
+    ap.add_argument("--c3-replays", type=int, default=3, help="timed C3 applies (replays)")
+    ap.add_argument("--c3-only", action="store_true", help="run only the C3 measurement (for profiling)")
     ap.add_argument("--c5-requests", type=int, default=30, help="requests per offered rate in C5")
     ap.add_argument("--c5-max", type=int, default=500000, help="largest triangle count of a C5 request")
     args = ap.parse_args()
     if args.kernel_pass:
         kernel_pass(args)
+        return
+    if args.c3_only:
+        print(json.dumps(c3_half(args)), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "the library runs on one GPU per process; --gpus must be 1"}))
@@ -477,6 +568,7 @@ def main():
             if r.get("kernel") in traffic:
                 r["traffic"] = traffic[r["kernel"]]
 
+    c3 = c3_half(args) if args.c3 else None
     c5 = c5_served(args) if args.c5 else None
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
@@ -499,6 +591,7 @@ def main():
         "phases_ms": ph, "kernel_phases_ms": kph, "work": stats,
         "roofline": roof, "rooflines": dict(rooflines, **({"iter_replay": replay} if replay else {})),
         "cpu_baseline": cpu,
+        "half_space_c3": c3,
         "served_c5": c5,
         "traffic": ({"bytes_per_dispatch": traffic, "counter_scale": factors,
                      "method": f"rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, over a child run of two applies; "

@@ -221,6 +221,47 @@ def elfmm3dtriatarg_dev(iprec, rlam, rmu, triangle, trinorm, source, sigma_sl, s
         C.c_int(straintarg is not None), ptr(straintarg), s)
 
 
+def elfmm3dtriatargiter_dev(iprec, rlam, rmu, triangle, trinorm, source, sigma_sl, sigma_dl, target,
+                            ptfrc, strain, ptfrctarg, straintarg, icomp_type, wsave, stream=None):
+    """Device-resident iter apply (elfmm_gfx950_triatargiter_dev) on torch
+    tensors (float64, contiguous); wsave is a numpy array (>= 4 words for a
+    store). Any of sigma_sl/sigma_dl/target/outputs may be None. Returns
+    (ier, lused)."""
+    def ptr(t):
+        return C.c_void_p(0 if t is None else t.data_ptr())
+    n = triangle.shape[0]
+    m = 0 if target is None else target.shape[0]
+    s = C.c_void_p(0 if stream is None else stream.cuda_stream)
+    lused = C.c_int(0)
+    ier = library().elfmm_gfx950_triatargiter_dev(
+        C.c_int(iprec), C.c_double(rlam), C.c_double(rmu), C.c_int(n), ptr(triangle), ptr(trinorm),
+        ptr(source), C.c_int(sigma_sl is not None), ptr(sigma_sl), C.c_int(sigma_dl is not None),
+        ptr(sigma_dl), C.c_int(ptfrc is not None), ptr(ptfrc), C.c_int(strain is not None),
+        ptr(strain), C.c_int(m), ptr(target), C.c_int(ptfrctarg is not None), ptr(ptfrctarg),
+        C.c_int(straintarg is not None), ptr(straintarg), C.c_int(icomp_type), _p(wsave), C.c_int(wsave.size),
+        C.byref(lused), s)
+    return ier, lused.value
+
+
+def elhfmm3dtriatargiter_dev(iprec, rlam, rmu, triangle, trinorm, source, sigma_sl, sigma_dl, target,
+                             ptfrctarg, straintarg, icomp_type, wsave, stream=None):
+    """Device-resident half-space image part (elfmm_gfx950_htriatargiter_dev)
+    on torch tensors. Returns (ier, lused)."""
+    def ptr(t):
+        return C.c_void_p(0 if t is None else t.data_ptr())
+    n = triangle.shape[0]
+    m = target.shape[0]
+    s = C.c_void_p(0 if stream is None else stream.cuda_stream)
+    lused = C.c_int(0)
+    ier = library().elfmm_gfx950_htriatargiter_dev(
+        C.c_int(iprec), C.c_double(rlam), C.c_double(rmu), C.c_int(n), ptr(triangle), ptr(trinorm),
+        ptr(source), C.c_int(sigma_sl is not None), ptr(sigma_sl), C.c_int(sigma_dl is not None),
+        ptr(sigma_dl), C.c_int(m), ptr(target), C.c_int(ptfrctarg is not None), ptr(ptfrctarg),
+        C.c_int(straintarg is not None), ptr(straintarg), C.c_int(icomp_type), _p(wsave), C.c_int(wsave.size),
+        C.byref(lused), s)
+    return ier, lused.value
+
+
 def last_stats():
     """Work counts of the last apply on this thread (see include/elfmm_gfx950.h)."""
     out = np.zeros(16)

@@ -192,3 +192,18 @@ def sphere_workload(nsplit=64, ntarg=100000, seed=2024):
     sdl = rng.uniform(-1, 1, (tri.shape[0], 3))
     tg = cube_targets(ntarg, 2.0, 1.0, rng)
     return tri, nrm, src, ssl, sdl, tg
+
+
+def strain_to_stress(strain, lam, mu):
+    """Isotropic Hooke's law, sigma = lam tr(eps) I + 2 mu eps, for strains of
+    shape (..., 3, 3) (el3dstrain2stress of the reference's MATLAB tools,
+    matlab/Contents.m:27)."""
+    strain = np.asarray(strain, dtype=np.float64)
+    tr = np.trace(strain, axis1=-2, axis2=-1)
+    return lam * tr[..., None, None] * np.eye(3) + 2.0 * mu * strain
+
+
+def stress_to_traction(stress, normal):
+    """Traction (stress vector) t_i = sigma_ij n_j for stresses (n, 3, 3) and
+    unit normals (n, 3) (matlab/el3dstress2traction.m)."""
+    return np.einsum("nij,nj->ni", np.asarray(stress, dtype=np.float64), np.asarray(normal, dtype=np.float64))

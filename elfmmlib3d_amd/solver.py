@@ -84,34 +84,72 @@ class IterOperator:
     single (layer="single") or double (layer="double") layer potential with
     density x, through the iter engine: the first call stores the
     near-field interactions (icomp 2), later calls replay them (icomp 3).
-    Call release() to free the device store."""
+    half=True adds the Mindlin half-space image part
+    (elhfmm3dtriatargiter at the centroids), as the reference's half-space
+    driver sums the two. device="cuda" keeps the geometry, the vectors and
+    the products on the GPU (torch tensors in, torch tensors out; only the
+    coordinates the host planner reads and the solver's scalars cross
+    PCIe); device=None works on numpy arrays through the host entry points.
+    Call release() to free the device stores."""
 
-    def __init__(self, iprec, rlam, rmu, triangle, trinorm, source, layer="double"):
+    def __init__(self, iprec, rlam, rmu, triangle, trinorm, source, layer="double", half=False, device=None):
         from . import _lib
         self._lib = _lib
-        self.args = (iprec, rlam, rmu, triangle, trinorm, source)
         self.layer = layer
-        ier, lused, *_ = _lib.elfmm3dtriatargiter(iprec, rlam, rmu, triangle, trinorm, source,
-                                                  *self._dens(np.zeros((triangle.shape[0], 3))),
-                                                  None, ifstrain=0, icomp_type=4)
-        if ier:
-            raise RuntimeError(f"elfmm3dtriatargiter ier={ier}")
-        self.wsave = np.zeros(lused)
+        self.half = half
+        self.device = device
+        self.n = triangle.shape[0]
+        self.iprec, self.rlam, self.rmu = iprec, rlam, rmu
+        self.wsave = np.zeros(4)
+        self.wsave_h = np.zeros(4)
         self.calls = 0
+        if device is not None:
+            import torch
+            self._torch = torch
+            T = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=device)
+            self.geo = (T(triangle), T(trinorm), T(source))
+            self.out = torch.zeros((self.n, 3), dtype=torch.float64, device=device)
+            self.out_h = torch.zeros((self.n, 3), dtype=torch.float64, device=device)
+        else:
+            self.geo = (triangle, trinorm, source)
 
     def _dens(self, s):
         return (s, None) if self.layer == "single" else (None, s)
 
     def __call__(self, x):
-        n = self.args[3].shape[0]
-        s = np.asarray(x, dtype=np.float64).reshape(n, 3)
         icomp = 2 if self.calls == 0 else 3
-        ier, _, pt, *_ = self._lib.elfmm3dtriatargiter(*self.args, *self._dens(s), None, ifstrain=0,
-                                                       icomp_type=icomp, wsave=self.wsave)
+        if self.device is None:
+            s = np.asarray(x, dtype=np.float64).reshape(self.n, 3)
+            args = (self.iprec, self.rlam, self.rmu, *self.geo, *self._dens(s))
+            ier, _, pt, *_ = self._lib.elfmm3dtriatargiter(*args, None, ifstrain=0, icomp_type=icomp,
+                                                           wsave=self.wsave)
+            if ier:
+                raise RuntimeError(f"elfmm3dtriatargiter ier={ier}")
+            if self.half:
+                ier, _, ph, _ = self._lib.elhfmm3dtriatargiter(*args, self.geo[2], 1, 0, icomp_type=icomp,
+                                                               wsave=self.wsave_h)
+                if ier:
+                    raise RuntimeError(f"elhfmm3dtriatargiter ier={ier}")
+                pt = pt + ph
+            self.calls += 1
+            return pt.reshape(-1)
+        torch = self._torch
+        s = torch.as_tensor(x, dtype=torch.float64, device=self.device).reshape(self.n, 3).contiguous()
+        args = (self.iprec, self.rlam, self.rmu, *self.geo, *self._dens(s))
+        ier, _ = self._lib.elfmm3dtriatargiter_dev(*args, None, self.out, None, None, None, icomp, self.wsave)
         if ier:
             raise RuntimeError(f"elfmm3dtriatargiter ier={ier}")
+        y = self.out.clone()
+        if self.half:
+            ier, _ = self._lib.elhfmm3dtriatargiter_dev(*args, self.geo[2], self.out_h, None, icomp, self.wsave_h)
+            if ier:
+                raise RuntimeError(f"elhfmm3dtriatargiter ier={ier}")
+            y += self.out_h
         self.calls += 1
-        return pt.reshape(-1)
+        return y.reshape(-1)
 
     def release(self):
-        return self._lib.iter_release(self.wsave)
+        r = self._lib.iter_release(self.wsave)
+        if self.half:
+            r += self._lib.iter_release(self.wsave_h)
+        return r

@@ -19,7 +19,11 @@ from ._lib import (  # noqa: F401
     elfmm3dtriatarg,
     elfmm3dtriatarg_dev,
     elfmm3dtriatargiter,
+    elfmm3dtriatargiter_dev,
+    elhfmm3dtriatargiter_dev,
+    expansion_order,
     iter_release,
+    triatargiter_multi_dev,
     last_stats,
     last_timings,
     library,
@@ -30,4 +34,4 @@ from ._build import build  # noqa: F401
 __all__ = ["elfmm3dtria", "elfmm3dtriatarg", "el3dtriadirecttarg", "elfmm3dtriatarg_dev",
            "elfmm3dtriatargiter", "elhfmm3dtriatargiter", "elh3dtriadirecttarg", "half_parts",
            "iter_release",
-           "last_stats", "last_timings", "library", "library_path", "build"]
+           "last_stats", "last_timings", "library", "library_path", "build", "elfmm3dtriatargiter_dev", "elhfmm3dtriatargiter_dev", "expansion_order", "triatargiter_multi_dev"]

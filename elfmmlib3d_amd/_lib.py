@@ -262,6 +262,31 @@ def elhfmm3dtriatargiter_dev(iprec, rlam, rmu, triangle, trinorm, source, sigma_
     return ier, lused.value
 
 
+def triatargiter_multi_dev(iprec, rlam, rmu, triangle, trinorm, source, sigma_sl, sigma_dl, target, ptfrc,
+                           strain, ptfrctarg, straintarg, wsave, half=False, stream=None):
+    """Replays of a store for several density sets in one call
+    (elfmm_gfx950_triatargiter_multi_dev, or the half-space image part with
+    half=True): torch tensors, densities (nrhs, n, 3), outputs (nrhs, ...).
+    Returns ier."""
+    def ptr(t):
+        return C.c_void_p(0 if t is None else t.data_ptr())
+    n = triangle.shape[0]
+    m = 0 if target is None else target.shape[0]
+    nrhs = (sigma_dl if sigma_dl is not None else sigma_sl).shape[0]
+    s = C.c_void_p(0 if stream is None else stream.cuda_stream)
+    common = [C.c_int(iprec), C.c_double(rlam), C.c_double(rmu), C.c_int(n), ptr(triangle), ptr(trinorm),
+              ptr(source), C.c_int(nrhs), C.c_int(sigma_sl is not None), ptr(sigma_sl),
+              C.c_int(sigma_dl is not None), ptr(sigma_dl)]
+    if half:
+        return library().elfmm_gfx950_htriatargiter_multi_dev(
+            *common, C.c_int(m), ptr(target), C.c_int(ptfrctarg is not None), ptr(ptfrctarg),
+            C.c_int(straintarg is not None), ptr(straintarg), _p(wsave), C.c_int(wsave.size), s)
+    return library().elfmm_gfx950_triatargiter_multi_dev(
+        *common, C.c_int(ptfrc is not None), ptr(ptfrc), C.c_int(strain is not None), ptr(strain), C.c_int(m),
+        ptr(target), C.c_int(ptfrctarg is not None), ptr(ptfrctarg), C.c_int(straintarg is not None),
+        ptr(straintarg), _p(wsave), C.c_int(wsave.size), s)
+
+
 def last_stats():
     """Work counts of the last apply on this thread (see include/elfmm_gfx950.h)."""
     out = np.zeros(16)

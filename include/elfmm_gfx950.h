@@ -97,6 +97,30 @@ int elfmm_gfx950_triatargiter_dev(int iprec, double rlam, double rmu, int nparts
                                   int ifstraintarg, double *d_straintarg, int icomp_type,
                                   double *wsave, int lwsave, int *lused, void *stream);
 
+/* Replays (icomp_type 3) of a store for nrhs density sets in one call: the
+ * near field of all sets in one pass over the store per 4 (both layers) or
+ * 8 sets, the far field of each set on the plan kept with the store.
+ * Densities and outputs are nrhs consecutive blocks of the single-set
+ * layout (d_sigma_dl[r * 3 * nparts ...], d_ptfrctarg[r * 3 * ntargs ...]).
+ * Returns 64 without a store of these coordinates (nothing computed). No
+ * counterpart in the reference (its callers loop over icomp 3 calls). */
+int elfmm_gfx950_triatargiter_multi_dev(int iprec, double rlam, double rmu, int nparts,
+                                        const double *d_triangle, const double *d_trinorm,
+                                        const double *d_source, int nrhs, int ifsingle,
+                                        const double *d_sigma_sl, int ifdouble, const double *d_sigma_dl,
+                                        int ifptfrc, double *d_ptfrc, int ifstrain, double *d_strain,
+                                        int ntargs, const double *d_target, int ifptfrctarg,
+                                        double *d_ptfrctarg, int ifstraintarg, double *d_straintarg,
+                                        double *wsave, int lwsave, void *stream);
+/* The same for the half-space image part (elhfmm3dtriatargiter's store). */
+int elfmm_gfx950_htriatargiter_multi_dev(int iprec, double rlam, double rmu, int nparts,
+                                         const double *d_triangle, const double *d_trinorm,
+                                         const double *d_source, int nrhs, int ifsingle,
+                                         const double *d_sigma_sl, int ifdouble, const double *d_sigma_dl,
+                                         int ntargs, const double *d_target, int ifptfrctarg,
+                                         double *d_ptfrctarg, int ifstraintarg, double *d_straintarg,
+                                         double *wsave, int lwsave, void *stream);
+
 /* Frees the device-resident interactions named by wsave (returns 1 if
  * freed). No counterpart in the reference, whose wsave is the caller's
  * memory.

@@ -227,16 +227,127 @@ def c3_half(args):
             tb.append(t2 - t1)
             tim.append(tm)
         apply_s = float(np.median(np.array(ta) + np.array(tb)))
+        multi = c3_multi_rhs(args, E, n, d_tri, d_nrm, d_src, wa, wb, arrival, image, pa, sa, ph, sh, rng)
         return {"workload": f"C3: 8 faults, {n} triangles, slip U[-1,1]^3, centroids, iprec {args.iprec}",
                 "value": n / apply_s / 1e6, "unit": "M tris/s", "apply_ms": apply_s * 1e3,
                 "arrival_replay_ms": float(np.median(ta)) * 1e3, "image_replay_ms": float(np.median(tb)) * 1e3,
                 "image_mindlin_ms": float(np.median([x["mindlin"] for x in tim])),
                 "store_s": {"arrival": t_store_a, "image": t_store_b},
                 "store_gb": {"arrival": words_a * 8 / 1e9, "image": words_b * 8 / 1e9},
-                "replays": args.c3_replays}
+                "replays": args.c3_replays, "multi_rhs": multi}
     finally:
         E.iter_release(wa)
         E.iter_release(wb)
+
+
+def c3_multi_rhs(args, E, n, d_tri, d_nrm, d_src, wa, wb, arrival, image, pa, sa, ph, sh, rng):
+    """C4's first half: args.c3_rhs slip vectors on the C3 mesh in one call
+    per part (triatargiter_multi_dev on both stores: near field of 4 sets
+    per pass over the store, far field per set), against the same sets
+    replayed one call at a time. Set 0 is checked against its single
+    replay."""
+    import torch
+    R = args.c3_rhs
+    if R < 2:
+        return None
+    dev = d_src.device
+    slips = torch.empty((R, n, 3), dtype=torch.float64, device=dev).uniform_(-1, 1)
+    z = lambda *sh_: torch.zeros(sh_, dtype=torch.float64, device=dev)
+    oa, osa, oh, osh = z(R, n, 3), z(R, n, 3, 3), z(R, n, 3), z(R, n, 3, 3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ier = E.triatargiter_multi_dev(args.iprec, 1.0, 1.0, d_tri, d_nrm, d_src, None, slips, d_src, None, None,
+                                   oa, osa, wa)
+    ier2 = E.triatargiter_multi_dev(args.iprec, 1.0, 1.0, d_tri, d_nrm, d_src, None, slips, d_src, None, None,
+                                    oh, osh, wb, half=True)
+    torch.cuda.synchronize()
+    t_multi = time.perf_counter() - t0
+    assert ier == 0 and ier2 == 0, (ier, ier2)
+    # the same sets one replay at a time (a sample of them, scaled)
+    ns = min(R, 8)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in range(ns):
+        assert arrival(3, slips[r].contiguous())[0] == 0
+        assert image(3, slips[r].contiguous())[0] == 0
+        if r == 0:
+            torch.cuda.synchronize()
+            e = max(float(torch.linalg.norm(a - b[0]) / torch.linalg.norm(a))
+                    for a, b in ((pa, oa), (sa, osa), (ph, oh), (sh, osh)))
+    torch.cuda.synchronize()
+    t_single = (time.perf_counter() - t0) / ns * R
+    return {"rhs": R, "multi_s": t_multi, "single_calls_s": t_single, "speedup": t_single / t_multi,
+            "value": R * n / t_multi / 1e6, "unit": "M tris x rhs / s", "set0_vs_single_rel": e,
+            "single_sampled_sets": ns}
+
+
+def c4_batched(args):
+    """C4's second half: args.c4_meshes independent icospheres (triangle
+    counts log-uniform in 500-20,000, radii 0.5-3, random centres; SL + DL
+    densities U[-1,1]^3; displacement and strain at the centroids; iprec
+    2) as one batch (tuned schedule, graph replay), against the plain path
+    (one elfmm3dtriatarg_dev call per mesh, back to back). Accuracy: a few
+    meshes against the direct sum."""
+    import torch
+    import elfmmlib3d_amd as E
+    from elfmmlib3d_amd import mesh
+    ws = mesh.sphere_batch(args.c4_meshes, 500, 20000, 2025)
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    meshes = [(T(w[0]), T(w[1]), T(w[2]), None) for w in ws]
+    ssl = T(np.concatenate([w[3] for w in ws]))
+    sdl = T(np.concatenate([w[4] for w in ws]))
+    counts = [w[0].shape[0] for w in ws]
+    N = sum(counts)
+    t0 = time.perf_counter()
+    B = E.MeshBatch(args.iprec, 1.0, 1.0, meshes, ifptfrctarg=False, ifstraintarg=False, width=0)
+    t_create = time.perf_counter() - t0
+    width, graph, tune = B.schedule()
+    outs = B.apply(ssl, sdl)
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        outs = B.apply(ssl, sdl)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / reps
+    # plain path: one call per mesh
+    z = lambda *sh_: torch.empty(sh_, dtype=torch.float64, device=dev)
+    po, so = z(N, 3), z(N, 3, 3)
+    sl, dl = ssl.split(counts), sdl.split(counts)
+    pos, sos = po.split(counts), so.split(counts)
+
+    def plain():
+        for k, (tri, nrm, src, _) in enumerate(meshes):
+            ier = E.elfmm3dtriatarg_dev(args.iprec, 1.0, 1.0, tri, nrm, src, sl[k], dl[k], None, pos[k], sos[k],
+                                        None, None)
+            assert ier == 0
+    plain()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain()
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    same = max(float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) for a, b in ((outs[0], po), (outs[1], so)))
+    # accuracy: the largest, the smallest and two random meshes vs the direct sum
+    order = np.argsort(counts)
+    pick = sorted({int(order[0]), int(order[-1]), int(order[len(order) // 3]), int(order[2 * len(order) // 3])})
+    off = np.concatenate([[0], np.cumsum(counts)])
+    pt, st = outs[0].cpu().numpy(), outs[1].cpu().numpy()
+    err = 0.0
+    for k in pick:
+        tri, nrm, src, s1, s2 = ws[k]
+        dp, ds_, _, _ = E.el3dtriadirecttarg(1.0, 1.0, tri, nrm, src, s1, s2, np.zeros((0, 3)), 1, 1, 0, 0)
+        a, b = pt[off[k]:off[k + 1]], st[off[k]:off[k + 1]]
+        err = max(err, float(np.linalg.norm(a - dp) / np.linalg.norm(dp)))
+    B.close()
+    return {"workload": f"C4 meshes: {len(ws)} icospheres, {N} triangles ({min(counts)}-{max(counts)} each), "
+                        f"SL+DL, centroids, iprec {args.iprec}",
+            "value": N / t_batch / 1e6, "unit": "M tris/s", "apply_ms": t_batch * 1e3,
+            "plain_ms": t_plain * 1e3, "speedup_vs_plain": t_plain / t_batch, "create_s": t_create,
+            "schedule": {"width": width, "graph": graph,
+                         "tuned": [{"graph": g, "width": w, "ms": ms} for g, w, ms in tune]},
+            "batch_vs_plain_rel": same, "disp_err_vs_direct": err, "checked_meshes": pick}
 
 
 def c5_served(args):
@@ -375,6 +486,10 @@ def main():
     ap.add_argument("--c3-replays", type=int, default=3, help="timed C3 applies (replays)")
     ap.add_argument("--c3-only", action="store_true", help="run only the C3 measurement (for profiling)")
     ap.add_argument("--c5-requests", type=int, default=30, help="requests per offered rate in C5")
+    ap.add_argument("--c3-rhs", type=int, default=64, help="slip vectors of the C3 multi-set call (0: skip)")
+    ap.add_argument("--no-c4", dest="c4", action="store_false", help="skip the C4 many-mesh measurement")
+    ap.add_argument("--c4-meshes", type=int, default=256, help="meshes of the C4 batch")
+    ap.add_argument("--c4-only", action="store_true", help="run only the C4 many-mesh measurement")
     ap.add_argument("--c5-max", type=int, default=500000, help="largest triangle count of a C5 request")
     args = ap.parse_args()
     if args.kernel_pass:
@@ -382,6 +497,9 @@ def main():
         return
     if args.c3_only:
         print(json.dumps(c3_half(args)), flush=True)
+        return
+    if args.c4_only:
+        print(json.dumps(c4_batched(args)), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "the library runs on one GPU per process; --gpus must be 1"}))
@@ -569,6 +687,7 @@ def main():
                 r["traffic"] = traffic[r["kernel"]]
 
     c3 = c3_half(args) if args.c3 else None
+    c4 = c4_batched(args) if args.c4 else None
     c5 = c5_served(args) if args.c5 else None
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
@@ -592,6 +711,7 @@ def main():
         "roofline": roof, "rooflines": dict(rooflines, **({"iter_replay": replay} if replay else {})),
         "cpu_baseline": cpu,
         "half_space_c3": c3,
+        "batched_c4": c4,
         "served_c5": c5,
         "traffic": ({"bytes_per_dispatch": traffic, "counter_scale": factors,
                      "method": f"rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, over a child run of two applies; "

@@ -11,6 +11,7 @@ if it is not built, every call raises.
 from __future__ import annotations
 
 from ._lib import (  # noqa: F401
+    MeshBatch,
     el3dtriadirecttarg,
     elh3dtriadirecttarg,
     elhfmm3dtriatargiter,
@@ -34,4 +35,5 @@ from ._build import build  # noqa: F401
 __all__ = ["elfmm3dtria", "elfmm3dtriatarg", "el3dtriadirecttarg", "elfmm3dtriatarg_dev",
            "elfmm3dtriatargiter", "elhfmm3dtriatargiter", "elh3dtriadirecttarg", "half_parts",
            "iter_release",
-           "last_stats", "last_timings", "library", "library_path", "build", "elfmm3dtriatargiter_dev", "elhfmm3dtriatargiter_dev", "expansion_order", "triatargiter_multi_dev"]
+           "last_stats", "last_timings", "library", "library_path", "build", "elfmm3dtriatargiter_dev", "elhfmm3dtriatargiter_dev", "expansion_order", "triatargiter_multi_dev",
+           "MeshBatch"]

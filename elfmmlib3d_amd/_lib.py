@@ -287,6 +287,77 @@ def triatargiter_multi_dev(iprec, rlam, rmu, triangle, trinorm, source, sigma_sl
         ptr(straintarg), _p(wsave), C.c_int(wsave.size), s)
 
 
+class MeshBatch:
+    """Many free-space meshes applied together (elfmm_gfx950_batch_*): each
+    mesh's result is its elfmm3dtriatarg. ``meshes`` is a list of
+    (triangle (n,3,3), trinorm (n,3), source (n,3), target (m,3) or None) as
+    torch float64 tensors on the current GPU; the geometry is copied at
+    creation. width 0 lets the library's tuner choose the schedule.
+    apply() takes packed densities (sum n, 3) and returns packed outputs
+    (ptfrc (sum n, 3), strain (sum n, 3, 3), ptfrctarg (sum m, 3),
+    straintarg (sum m, 3, 3); None where off); split() cuts a packed array
+    into per-mesh pieces."""
+
+    def __init__(self, iprec, rlam, rmu, meshes, ifsingle=True, ifdouble=True, ifptfrc=True, ifstrain=True,
+                 ifptfrctarg=True, ifstraintarg=True, width=0):
+        import torch
+        cat = lambda ts: torch.cat([t.reshape(t.shape[0], -1) for t in ts]).contiguous()
+        self.n = [int(m[0].shape[0]) for m in meshes]
+        self.m = [0 if m[3] is None else int(m[3].shape[0]) for m in meshes]
+        self.flags = (bool(ifsingle), bool(ifdouble), bool(ifptfrc), bool(ifstrain), bool(ifptfrctarg) and
+                      sum(self.m) > 0, bool(ifstraintarg) and sum(self.m) > 0)
+        tri, nrm, src = cat([m[0] for m in meshes]), cat([m[1] for m in meshes]), cat([m[2] for m in meshes])
+        tg = cat([m[3] for m in meshes if m[3] is not None]) if sum(self.m) else None
+        nparts = (C.c_int * len(meshes))(*self.n)
+        ntargs = (C.c_int * len(meshes))(*self.m)
+        h = C.c_void_p()
+        f = self.flags
+        ier = library().elfmm_gfx950_batch_create(
+            C.c_int(iprec), C.c_double(rlam), C.c_double(rmu), C.c_int(len(meshes)), nparts,
+            C.c_void_p(tri.data_ptr()), C.c_void_p(nrm.data_ptr()), C.c_void_p(src.data_ptr()), C.c_int(f[0]),
+            C.c_int(f[1]), C.c_int(f[2]), C.c_int(f[3]), ntargs, C.c_void_p(0 if tg is None else tg.data_ptr()),
+            C.c_int(f[4]), C.c_int(f[5]), C.c_int(width), C.byref(h))
+        if ier:
+            raise RuntimeError(f"elfmm_gfx950_batch_create failed: ier={ier}")
+        self.handle = h
+
+    def apply(self, sigma_sl=None, sigma_dl=None, stream=None):
+        import torch
+        N, M = sum(self.n), sum(self.m)
+        z = lambda *sh: torch.empty(sh, dtype=torch.float64, device="cuda")
+        f = self.flags
+        outs = [z(N, 3) if f[2] else None, z(N, 3, 3) if f[3] else None, z(M, 3) if f[4] else None,
+                z(M, 3, 3) if f[5] else None]
+        ptr = lambda t: C.c_void_p(0 if t is None else t.data_ptr())
+        s = C.c_void_p(0 if stream is None else stream.cuda_stream)
+        ier = library().elfmm_gfx950_batch_apply(self.handle, ptr(sigma_sl), ptr(sigma_dl), *map(ptr, outs), s)
+        if ier:
+            raise RuntimeError(f"elfmm_gfx950_batch_apply failed: ier={ier}")
+        return outs
+
+    def split(self, a, targets=False):
+        counts = self.m if targets else self.n
+        return list(a.split(counts)) if a is not None else None
+
+    def schedule(self):
+        """(width, graph, [(graph, width, ms), ...]) of the tuner."""
+        w, g = C.c_int(), C.c_int()
+        tune = (C.c_double * 96)()
+        k = library().elfmm_gfx950_batch_schedule(self.handle, C.byref(w), C.byref(g), 32, tune)
+        return w.value, g.value, [(int(tune[3 * i]), int(tune[3 * i + 1]), tune[3 * i + 2]) for i in range(k)]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            library().elfmm_gfx950_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def last_stats():
     """Work counts of the last apply on this thread (see include/elfmm_gfx950.h)."""
     out = np.zeros(16)

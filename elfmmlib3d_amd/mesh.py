@@ -194,6 +194,29 @@ def sphere_workload(nsplit=64, ntarg=100000, seed=2024):
     return tri, nrm, src, ssl, sdl, tg
 
 
+def sphere_batch(nmesh=256, tmin=500, tmax=20000, seed=2025, box=20.0):
+    """The many-mesh workload: nmesh independent icospheres whose triangle
+    counts 20*nsplit**2 follow a log-uniform draw in [tmin, tmax] (nsplit
+    rounded, at least 1), radii uniform in [0.5, 3], centres uniform in
+    [-box, box]^3, densities uniform in [-1,1]^3. Returns a list of
+    (triangle, trinorm, source, sigma_sl, sigma_dl)."""
+    rng = np.random.default_rng(seed)
+    unit = {}
+    out = []
+    for _ in range(int(nmesh)):
+        cnt = np.exp(rng.uniform(np.log(tmin), np.log(tmax)))
+        ns = max(1, int(round(np.sqrt(cnt / 20.0))))
+        r = rng.uniform(0.5, 3.0)
+        c = rng.uniform(-box, box, 3)
+        if ns not in unit:
+            unit[ns] = icosphere(ns)
+        tri = np.ascontiguousarray(unit[ns] * r + c)
+        n = tri.shape[0]
+        out.append((tri, triangle_normals(tri), centroids(tri), rng.uniform(-1, 1, (n, 3)),
+                    rng.uniform(-1, 1, (n, 3))))
+    return out
+
+
 def strain_to_stress(strain, lam, mu):
     """Isotropic Hooke's law, sigma = lam tr(eps) I + 2 mu eps, for strains of
     shape (..., 3, 3) (el3dstrain2stress of the reference's MATLAB tools,

@@ -227,6 +227,48 @@ int elfmm_gfx950_triatarg_dev(int iprec, double rlam, double rmu, int nparts,
                               const double *d_target, int ifptfrctarg, double *d_ptfrctarg,
                               int ifstraintarg, double *d_straintarg, void *stream);
 
+/* Batch of meshes: nmesh independent free-space applies, each the
+ * elfmm3dtriatarg of one mesh (src/elfmm3dtria.f:57-61 per mesh, same
+ * precision flag, Lame constants, layers and output flags for all), on
+ * geometries fixed at creation. Arrays are packed mesh after mesh in the
+ * caller's order: mesh m owns nparts[m] triangles (d_triangle 9 doubles,
+ * d_trinorm, d_source, densities and d_ptfrc 3, d_strain 9 per triangle)
+ * and ntargs[m] targets (ntargs may be NULL: no targets). The geometry is
+ * copied; octrees, task lists and device buffers are built once. width in
+ * 1..16 fixes the number of branches (concurrent chains of meshes) and
+ * replays one graph built for them; -16..-1 fixes -width branches launched
+ * directly on streams; 0 lets the tuner time widths 1, 2, 4, 8, 16 both
+ * ways and keep the fastest. Returns ier
+ * (16 bad arguments, 4 tree failure, 8 host memory, 32 device error); on
+ * success *batch is the handle. No counterpart in the reference, whose
+ * callers loop over meshes. */
+int elfmm_gfx950_batch_create(int iprec, double rlam, double rmu, int nmesh, const int *nparts,
+                              const double *d_triangle, const double *d_trinorm,
+                              const double *d_source, int ifsingle, int ifdouble, int ifptfrc,
+                              int ifstrain, const int *ntargs, const double *d_target,
+                              int ifptfrctarg, int ifstraintarg, int width, void **batch);
+/* The batch's own packed density and output arrays (NULL where a layer or
+ * output is off): densities written there need no copy in, outputs read
+ * there need no copy out. */
+int elfmm_gfx950_batch_buffers(void *batch, double **d_sigma_sl, double **d_sigma_dl,
+                               double **d_ptfrc, double **d_strain, double **d_ptfrctarg,
+                               double **d_straintarg);
+/* One apply of every mesh, enqueued on stream and returning at once: the
+ * densities (packed; NULL or the batch's own array: no copy) are copied
+ * in, all meshes evaluated, and the outputs copied to the given arrays
+ * (NULL or the batch's own: no copy). Applies of one batch share its
+ * buffers: issue them on one stream (or synchronize between them). Must be
+ * called with the batch's device current. Returns ier. */
+int elfmm_gfx950_batch_apply(void *batch, const double *d_sigma_sl, const double *d_sigma_dl,
+                             double *d_ptfrc, double *d_strain, double *d_ptfrctarg,
+                             double *d_straintarg, void *stream);
+/* The schedule in use (branches, 1 = graph replay / 0 = direct launches)
+ * and up to ntune (graph, width, milliseconds) triples of the schedules
+ * the tuner timed; returns the number of schedules timed. */
+int elfmm_gfx950_batch_schedule(void *batch, int *width, int *graph, int ntune, double *tune);
+/* Releases the batch (waits for the device first). */
+int elfmm_gfx950_batch_destroy(void *batch);
+
 /* Per-phase times (milliseconds) of the last call on this thread:
  * [0] tree+plan (host), [1] upload, [2] P2M, [3] M2M, [4] M2L, [5] P2L,
  * [6] L2L, [7] L2T, [8] M2T, [9] near field, [10] assembly, [11] total

@@ -9,7 +9,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libelfmm_gfx950.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["elfmm_gpu.hip", "elfmm_half.hip", "elfmm_mindlin.hip", "elfmm_tree.cpp", "elfmm_plan.cpp", "elfmm_ops.cpp", "elfmm_pool.cpp"]
+SOURCES = ["elfmm_gpu.hip", "elfmm_half.hip", "elfmm_mindlin.hip", "elfmm_octree.hip", "elfmm_tree.cpp", "elfmm_plan.cpp", "elfmm_ops.cpp", "elfmm_pool.cpp"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions are disabled for all device code

@@ -269,6 +269,25 @@ int elfmm_gfx950_batch_schedule(void *batch, int *width, int *graph, int ntune, 
 /* Releases the batch (waits for the device first). */
 int elfmm_gfx950_batch_destroy(void *batch);
 
+/* The adaptive octree and interaction lists of the reference's d3tstrcr
+ * (src/d3tstrcr.f:31-266, lists :702-1179; imindlin != 0: the half-space
+ * root of src/d3tstrcr_mindlin.f:1380-1383) for nsource sources and ntarget
+ * targets (device arrays, 3 doubles per point; a box splits above nbox
+ * sources or targets), built on the GPU: the applies build their trees this
+ * way. Returns the number of boxes, or -4 when a box below level 21 would
+ * split (the applies then build that tree on the host), -16 bad arguments,
+ * -32 device error. sizes (7 ints, may be NULL): boxes, levels, entries of
+ * lists 1..5. The tree is kept for elfmm_gfx950_octree_get on this thread. */
+int elfmm_gfx950_octree_dev(const double *d_source, int nsource, const double *d_target, int ntarget,
+                            int nbox, int imindlin, int *sizes, void *stream);
+/* The last octree of this thread, 0-based: boxes (18 ints per box: level,
+ * i, j, k (1-based cube coordinates), parent (-1 for the root), number of
+ * children, first source, sources, first target, targets, the 8 child
+ * slots, -1 where empty), centers (3 doubles per box), iz / iztarg (input
+ * index of each point in tree order), lstart ((boxes + 1) ints per list,
+ * lists 1..5 in turn) and litems (their entries, lists 1..5 in turn). */
+void elfmm_gfx950_octree_get(int *boxes, double *centers, int *iz, int *iztarg, int *lstart, int *litems);
+
 /* Per-phase times (milliseconds) of the last call on this thread:
  * [0] tree+plan (host), [1] upload, [2] P2M, [3] M2M, [4] M2L, [5] P2L,
  * [6] L2L, [7] L2T, [8] M2T, [9] near field, [10] assembly, [11] total

@@ -47,6 +47,25 @@ extern "C" void tree_get(int *boxes, double *centers, int *iz, int *iztarg, int 
     }
 }
 
+// boxes[nb][18]: level, i, j, k, parent, nkids, src_start, src_count,
+// tgt_start, tgt_count, kids[8] (the layout of elfmm_gfx950_octree_get)
+extern "C" void tree_boxes_full(int *boxes)
+{
+    const elfmm::Tree &T = g_tree;
+    for (int b = 0; b < T.nboxes; b++) {
+        int *o = boxes + 18 * b;
+        o[0] = T.level[b];
+        for (int d = 0; d < 3; d++) o[1 + d] = T.ijk[3 * b + d];
+        o[4] = T.parent[b];
+        o[5] = T.nkids[b];
+        o[6] = T.src_start[b];
+        o[7] = T.src_count[b];
+        o[8] = T.tgt_start[b];
+        o[9] = T.tgt_count[b];
+        for (int q = 0; q < 8; q++) o[10 + q] = T.kids[8 * b + q];
+    }
+}
+
 // Near-field store size of the iter engine in the reference's word layout
 // (its lused for icomp 4): plan the iter near lists on the tree just built.
 extern "C" double plan_iter_ref_words(int n, int ifsl, int ifdl, int ifpt, int ifst, int ifptt, int ifstt)

@@ -215,12 +215,21 @@ __global__ void k_oct_boxes(BoxArrays B, const uint64_t *__restrict__ K, const i
     cur[i] = b;
 }
 
-__global__ void k_oct_counts(BoxArrays B, const int *__restrict__ P, const int *__restrict__ active,
-                             const int *__restrict__ cur, int N, int n)
+// counts of a new box from its last point: the points of a box are one
+// run of the sorted order, starting at ss + ts (sources and targets before
+// it), so the counts follow from the running source count
+__global__ void k_oct_counts(BoxArrays B, const uint64_t *__restrict__ K, const int *__restrict__ active,
+                             const int *__restrict__ cur, const int *__restrict__ tag,
+                             const int *__restrict__ sexcl, int N, int sh)
 {
     int i = blockIdx.x * OCT_T + threadIdx.x;
     if (i >= N || !active[i]) return;
-    atomicAdd(P[i] < n ? &B.sc[cur[i]] : &B.tc[cur[i]], 1);
+    bool tail = i == N - 1 || !active[i + 1] || (K[i] >> sh) != (K[i + 1] >> sh);
+    if (!tail) return;
+    int b = cur[i];
+    int s_upto = sexcl[i] + tag[i];                  // sources up to and including point i
+    B.sc[b] = s_upto - B.ss[b];
+    B.tc[b] = (i + 1 - s_upto) - B.ts[b];
 }
 
 __global__ void k_oct_split(BoxArrays B, const int *__restrict__ head, const int *__restrict__ cur, int N,
@@ -274,31 +283,46 @@ __global__ void k_oct_iota(int *__restrict__ a, int n)
 
 // ---- lists
 
-// colleagues (list 5) and list 2 of the boxes of one level
-__global__ void k_oct_coll(BoxArrays B, int first, int count, int *__restrict__ n5, int *__restrict__ l5,
-                           int *__restrict__ n2, int *__restrict__ l2)
+// colleagues (list 5) and list 2 of the boxes of one level, one wave per
+// box: the candidates (children of the parent and of the parent's
+// colleagues, in colleague then slot order) are taken 64 at a time and
+// compacted in order with ballots
+__global__ __launch_bounds__(OCT_T) void k_oct_coll(BoxArrays B, int first, int count, int *__restrict__ n5,
+                                                    int *__restrict__ l5, int *__restrict__ n2, int *__restrict__ l2)
 {
-    int t = blockIdx.x * OCT_T + threadIdx.x;
-    if (t >= count) return;
+    int t = (blockIdx.x * OCT_T + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (t >= count) return;                          // whole waves leave together
     int b = first + t;
     int dad = B.parent[b];
     int ncoll = 1 + n5[dad];
+    const int i0 = B.ijk[3 * b], i1 = B.ijk[3 * b + 1], i2 = B.ijk[3 * b + 2];
+    const uint64_t below = (1ull << lane) - 1;
     int c5 = 0, c2 = 0;
-    const int *ib = &B.ijk[3 * b];
-    for (int ci = 0; ci < ncoll; ci++) {
-        int coll = ci == 0 ? dad : l5[OCT_COLL * (size_t)dad + ci - 1];
-        for (int q = 0; q < B.nkids[coll]; q++) {
-            int kid = B.kids[8 * coll + q];
-            if (kid == b) continue;
-            const int *ik = &B.ijk[3 * kid];
-            if (abs(ik[0] - ib[0]) <= 1 && abs(ik[1] - ib[1]) <= 1 && abs(ik[2] - ib[2]) <= 1)
-                l5[OCT_COLL * (size_t)b + c5++] = kid;
-            else
-                l2[OCT_L2 * (size_t)b + c2++] = kid;
+    for (int f0 = 0; f0 < 8 * ncoll; f0 += 64) {
+        int f = f0 + lane, ci = f >> 3, q = f & 7;
+        bool valid = false, near = false;
+        int kid = -1;
+        if (ci < ncoll) {
+            int coll = ci == 0 ? dad : l5[OCT_COLL * (size_t)dad + ci - 1];
+            if (q < B.nkids[coll]) {
+                kid = B.kids[8 * coll + q];
+                if (kid != b) {
+                    valid = true;
+                    const int *ik = &B.ijk[3 * kid];
+                    near = abs(ik[0] - i0) <= 1 && abs(ik[1] - i1) <= 1 && abs(ik[2] - i2) <= 1;
+                }
+            }
         }
+        uint64_t m5 = __ballot(valid && near), m2 = __ballot(valid && !near);
+        if (valid && near) l5[OCT_COLL * (size_t)b + c5 + __popcll(m5 & below)] = kid;
+        if (valid && !near) l2[OCT_L2 * (size_t)b + c2 + __popcll(m2 & below)] = kid;
+        c5 += __popcll(m5);
+        c2 += __popcll(m2);
     }
-    n5[b] = c5;
-    n2[b] = c2;
+    if (lane == 0) {
+        n5[b] = c5;
+        n2[b] = c2;
+    }
 }
 
 __device__ __forceinline__ bool oct_touch(const BoxArrays &B, int a, int b)
@@ -314,67 +338,76 @@ __device__ __forceinline__ bool oct_touch(const BoxArrays &B, int a, int b)
     return true;
 }
 
-// lists 1 and 3 of the leaves by the reference's descent; fill = 0 counts
-// (c1, c3, ca per box), fill = 1 writes own list-1 entries (t1 at o1),
-// list 3 (l3 at o3, with (j, b) pairs p4 for list 4) and the (fine, coarse)
-// pairs pa at oa
-__global__ void k_oct_desc(BoxArrays B, int nb, const int *__restrict__ n5, const int *__restrict__ l5, int fill,
-                           int *__restrict__ c1, int *__restrict__ c3, int *__restrict__ ca,
+// lists 1 and 3 of the leaves by the reference's descent, one thread per
+// (box, colleague) entry q of the colleague CSR (owner[q] = box): fill = 0
+// counts (c1, c3, ca per entry), fill = 1 writes own list-1 entries (t1 at
+// o1), list 3 (l3 at o3, with (j, b) pairs p4 for list 4) and the (fine,
+// coarse) pairs pa at oa. Entries of one box are consecutive, in colleague
+// order, so the exclusive sums over entries give each box's lists in the
+// reference's order.
+__global__ __launch_bounds__(OCT_T) void k_oct_desc(BoxArrays B, int nq, const int *__restrict__ owner, const int *__restrict__ coll,
+                           int fill, int *__restrict__ c1, int *__restrict__ c3, int *__restrict__ ca,
                            const int *__restrict__ o1, const int *__restrict__ o3, const int *__restrict__ oa,
                            int *__restrict__ t1, int *__restrict__ l3, uint64_t *__restrict__ p4,
                            uint64_t *__restrict__ pa)
 {
-    int b = blockIdx.x * OCT_T + threadIdx.x;
-    if (b >= nb) return;
+    int q = blockIdx.x * OCT_T + threadIdx.x;
+    if (q >= nq) return;
+    int b = owner[q];
     int k1 = 0, k3 = 0, ka = 0;
     if (B.nkids[b] == 0 && B.level[b] > 0) {
-        int sbox[OCT_D + 3], sleft[OCT_D + 3];
-        for (int k = 0; k < n5[b]; k++) {
-            int c0 = l5[OCT_COLL * (size_t)b + k];
-            int sp = 0;
-            sbox[sp] = c0;
-            sleft[sp] = B.nkids[c0];
-            sp++;
-            while (sp > 0) {
-                int j = sbox[sp - 1];
+        // the descent of the reference without a stack: children are
+        // consecutive boxes, so the next one to visit (the previous slot)
+        // and the way back up follow from the parent links
+        const int c0 = coll[q];
+        int j = c0;
+        bool down = true;                            // j is to be examined (else: j is finished)
+        while (true) {
+            if (down) {
                 if (!oct_touch(B, b, j)) {
                     if (fill) {
-                        l3[o3[b] + k3] = j;
-                        p4[o3[b] + k3] = ((uint64_t)j << 32) | (uint32_t)b;
+                        l3[o3[q] + k3] = j;
+                        p4[o3[q] + k3] = ((uint64_t)j << 32) | (uint32_t)b;
                     }
                     k3++;
-                    sp--;
-                    if (sp > 0) sleft[sp - 1]--;
-                    continue;
-                }
-                if (B.nkids[j] == 0) {
-                    if (fill) t1[o1[b] + k1] = j;
+                    down = false;
+                } else if (B.nkids[j] == 0) {
+                    if (fill) t1[o1[q] + k1] = j;
                     k1++;
                     if (B.level[j] != B.level[b]) {
-                        if (fill) pa[oa[b] + ka] = ((uint64_t)j << 32) | (uint32_t)b;
+                        if (fill) pa[oa[q] + ka] = ((uint64_t)j << 32) | (uint32_t)b;
                         ka++;
                     }
-                    sp--;
-                    if (sp > 0) sleft[sp - 1]--;
-                    continue;
+                    down = false;
+                } else {
+                    j = B.kids[8 * j + B.nkids[j] - 1];   // last child first
                 }
-                if (sleft[sp - 1] == 0) {
-                    sp--;
-                    if (sp > 0) sleft[sp - 1]--;
-                    continue;
-                }
-                int kid = B.kids[8 * j + sleft[sp - 1] - 1];
-                sbox[sp] = kid;
-                sleft[sp] = B.nkids[kid];
-                sp++;
+                continue;
+            }
+            if (j == c0) break;
+            int par = B.parent[j];
+            int slot = j - B.kids[8 * par];
+            if (slot > 0) {
+                j = B.kids[8 * par + slot - 1];
+                down = true;
+            } else {
+                j = par;                                  // all children of par done
             }
         }
     }
     if (!fill) {
-        c1[b] = k1;
-        c3[b] = k3;
-        ca[b] = ka;
+        c1[q] = k1;
+        c3[q] = k3;
+        ca[q] = ka;
     }
+}
+
+// per-box starts from per-entry starts: box b's entries begin at st5[b]
+__global__ void k_oct_boxstart(int nb, const int *__restrict__ st5, const int *__restrict__ oq,
+                               int *__restrict__ ob)
+{
+    int b = blockIdx.x * OCT_T + threadIdx.x;
+    if (b <= nb) ob[b] = oq[st5[b]];
 }
 
 // count of sorted pairs per first box
@@ -394,13 +427,12 @@ __global__ void k_oct_list1_app(const uint64_t *__restrict__ pa, int np, const i
     items1[start1[j] + (k - aexcl[j])] = (int)(pa[k] & 0xffffffffu);
 }
 
-__global__ void k_oct_list1_own(int nb, const int *__restrict__ c1, const int *__restrict__ o1,
-                                const int *__restrict__ acnt, const int *__restrict__ t1,
-                                const int *__restrict__ start1, int *__restrict__ items1)
+__global__ void k_oct_list1_own(int nb, const int *__restrict__ o1, const int *__restrict__ acnt,
+                                const int *__restrict__ t1, const int *__restrict__ start1, int *__restrict__ items1)
 {
     int b = blockIdx.x * OCT_T + threadIdx.x;
     if (b >= nb) return;
-    for (int t = 0; t < c1[b]; t++) items1[start1[b] + acnt[b] + t] = t1[o1[b] + t];
+    for (int t = 0; t < o1[b + 1] - o1[b]; t++) items1[start1[b] + acnt[b] + t] = t1[o1[b] + t];
 }
 
 __global__ void k_oct_low32(const uint64_t *__restrict__ p, int np, int *__restrict__ out)
@@ -409,19 +441,24 @@ __global__ void k_oct_low32(const uint64_t *__restrict__ p, int np, int *__restr
     if (k < np) out[k] = (int)(p[k] & 0xffffffffu);
 }
 
-// rows of a fixed-stride table into CSR
+// rows of a fixed-stride table into CSR (owner, if given: the row of each
+// entry)
 __global__ void k_oct_rows(int nb, int stride, const int *__restrict__ cnt, const int *__restrict__ tab,
-                           const int *__restrict__ start, int *__restrict__ items)
+                           const int *__restrict__ start, int *__restrict__ items, int *__restrict__ owner)
 {
     int b = blockIdx.x * OCT_T + threadIdx.x;
     if (b >= nb) return;
-    for (int t = 0; t < cnt[b]; t++) items[start[b] + t] = tab[(size_t)stride * b + t];
+    for (int t = 0; t < cnt[b]; t++) {
+        items[start[b] + t] = tab[(size_t)stride * b + t];
+        if (owner) owner[start[b] + t] = b;
+    }
 }
 
-__global__ void k_oct_add(int nb, const int *__restrict__ a, const int *__restrict__ b, int *__restrict__ c)
+// list-1 length: appended entries plus own entries
+__global__ void k_oct_cnt1(int nb, const int *__restrict__ acnt, const int *__restrict__ o1, int *__restrict__ c)
 {
     int i = blockIdx.x * OCT_T + threadIdx.x;
-    if (i < nb) c[i] = a[i] + b[i];
+    if (i < nb) c[i] = acnt[i] + o1[i + 1] - o1[i];
     if (i == nb) c[i] = 0;
 }
 
@@ -487,7 +524,7 @@ struct OctreeWork::Impl {
     Buf part, root, key, key2, idx, idx2, tag, sexcl, head, incl, active, cur, dcount, tmp;
     Buf box[11];                                    // level, ijk, parent, kids, nkids, ss, sc, ts, tc, split, firstkid
     Buf sleaf, tleaf, iota, skey, iz, izt;
-    Buf n5, l5, n2, l2, c1, c3, ca, o1, o3, oa, t1, p4, p4s, pa, pas, acnt, aexcl, cnt4, cnt1;
+    Buf n5, l5, n2, l2, owner, c1, c3, ca, o1q, o3q, oaq, o1, t1, p4, p4s, pa, pas, acnt, aexcl, cnt4, cnt1;
     Buf st[6], it[6];
     size_t cap = 0;                                 // boxes the box arrays hold
 };
@@ -598,8 +635,8 @@ hipError_t build_tree_dev(const double *d_src, int n, const double *d_tgt, int n
         hipLaunchKernelGGL(k_oct_boxes, dim3(blocks(N)), dim3(OCT_T), 0, s, B, K, W.active.get<int>(),
                            W.head.get<int>(), W.incl.get<int>(), W.sexcl.get<int>(), N, lev, sh, nb, W.cur.get<int>(),
                            W.dcount.get<int>());
-        hipLaunchKernelGGL(k_oct_counts, dim3(blocks(N)), dim3(OCT_T), 0, s, B, P, W.active.get<int>(),
-                           W.cur.get<int>(), N, n);
+        hipLaunchKernelGGL(k_oct_counts, dim3(blocks(N)), dim3(OCT_T), 0, s, B, K, W.active.get<int>(),
+                           W.cur.get<int>(), W.tag.get<int>(), W.sexcl.get<int>(), N, sh);
         hipLaunchKernelGGL(k_oct_split, dim3(blocks(N)), dim3(OCT_T), 0, s, B, W.head.get<int>(), W.cur.get<int>(),
                            N, nbox, W.dcount.get<int>());
         hipLaunchKernelGGL(k_oct_kids, dim3(blocks(N)), dim3(OCT_T), 0, s, B, W.head.get<int>(), W.cur.get<int>(),
@@ -636,42 +673,68 @@ hipError_t build_tree_dev(const double *d_src, int n, const double *d_tgt, int n
     OCTOK(hipMemsetAsync(W.n2.p, 0, 4, s));
     for (int lev = 1; lev <= nlev; lev++) {
         int f = laddr[2 * lev], c = laddr[2 * lev + 1];
-        hipLaunchKernelGGL(k_oct_coll, dim3(blocks(c)), dim3(OCT_T), 0, s, B, f, c, W.n5.get<int>(), W.l5.get<int>(),
-                           W.n2.get<int>(), W.l2.get<int>());
+        hipLaunchKernelGGL(k_oct_coll, dim3(blocks((size_t)c * 64)), dim3(OCT_T), 0, s, B, f, c, W.n5.get<int>(),
+                           W.l5.get<int>(), W.n2.get<int>(), W.l2.get<int>());
     }
     OCTOK(hipGetLastError());
 
-    // lists 1 and 3 (and the pairs of the appends and of list 4)
-    for (OctreeWork::Impl::Buf *b : {&W.c1, &W.c3, &W.ca, &W.o1, &W.o3, &W.oa, &W.acnt, &W.aexcl, &W.cnt4, &W.cnt1})
-        OCTOK(b->ensure((size_t)(nb + 1) * 4));
-    hipLaunchKernelGGL(k_oct_desc, dim3(blocks(nb)), dim3(OCT_T), 0, s, B, nb, W.n5.get<int>(), W.l5.get<int>(), 0,
-                       W.c1.get<int>(), W.c3.get<int>(), W.ca.get<int>(), nullptr, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, nullptr);
+    // lists 5 and 2 from their tables (counts with a trailing zero); each
+    // list-5 entry also names its box, for the descent below
+    for (int t = 1; t <= 5; t++) OCTOK(W.st[t].ensure((size_t)(nb + 1) * 4));
+    OCTOK(hipMemsetAsync(W.n5.get<int>() + nb, 0, 4, s));
+    OCTOK(hipMemsetAsync(W.n2.get<int>() + nb, 0, 4, s));
+    OCTOK(cub_excl(W.n5.get<int>(), W.st[5].get<int>(), nb + 1));
+    OCTOK(cub_excl(W.n2.get<int>(), W.st[2].get<int>(), nb + 1));
+    int t5 = 0, t2 = 0;
+    OCTOK(hipMemcpyAsync(&t5, W.st[5].get<int>() + nb, 4, hipMemcpyDeviceToHost, s));
+    OCTOK(hipMemcpyAsync(&t2, W.st[2].get<int>() + nb, 4, hipMemcpyDeviceToHost, s));
+    OCTOK(hipStreamSynchronize(s));
+    OCTOK(W.it[5].ensure((size_t)t5 * 4 + 4));
+    OCTOK(W.it[2].ensure((size_t)t2 * 4 + 4));
+    OCTOK(W.owner.ensure((size_t)t5 * 4 + 4));
+    hipLaunchKernelGGL(k_oct_rows, dim3(blocks(nb)), dim3(OCT_T), 0, s, nb, OCT_COLL, W.n5.get<int>(), W.l5.get<int>(),
+                       W.st[5].get<int>(), W.it[5].get<int>(), W.owner.get<int>());
+    hipLaunchKernelGGL(k_oct_rows, dim3(blocks(nb)), dim3(OCT_T), 0, s, nb, OCT_L2, W.n2.get<int>(), W.l2.get<int>(),
+                       W.st[2].get<int>(), W.it[2].get<int>(), nullptr);
     OCTOK(hipGetLastError());
-    // counts with a trailing zero, so that the exclusive sums end in the totals
-    for (OctreeWork::Impl::Buf *b : {&W.c1, &W.c3, &W.ca}) OCTOK(hipMemsetAsync(b->get<int>() + nb, 0, 4, s));
-    OCTOK(cub_excl(W.c1.get<int>(), W.o1.get<int>(), nb + 1));
-    OCTOK(cub_excl(W.c3.get<int>(), W.o3.get<int>(), nb + 1));
-    OCTOK(cub_excl(W.ca.get<int>(), W.oa.get<int>(), nb + 1));
+
+    // lists 1 and 3 (and the pairs of the appends and of list 4), per
+    // list-5 entry; counts with a trailing zero
+    for (OctreeWork::Impl::Buf *b : {&W.c1, &W.c3, &W.ca, &W.o1q, &W.o3q, &W.oaq})
+        OCTOK(b->ensure((size_t)(t5 + 1) * 4));
+    for (OctreeWork::Impl::Buf *b : {&W.o1, &W.acnt, &W.aexcl, &W.cnt4, &W.cnt1}) OCTOK(b->ensure((size_t)(nb + 1) * 4));
+    if (t5)
+        hipLaunchKernelGGL(k_oct_desc, dim3(blocks(t5)), dim3(OCT_T), 0, s, B, t5, W.owner.get<int>(),
+                           W.it[5].get<int>(), 0, W.c1.get<int>(), W.c3.get<int>(), W.ca.get<int>(), nullptr, nullptr,
+                           nullptr, nullptr, nullptr, nullptr, nullptr);
+    OCTOK(hipGetLastError());
+    for (OctreeWork::Impl::Buf *b : {&W.c1, &W.c3, &W.ca}) OCTOK(hipMemsetAsync(b->get<int>() + t5, 0, 4, s));
+    OCTOK(cub_excl(W.c1.get<int>(), W.o1q.get<int>(), t5 + 1));
+    OCTOK(cub_excl(W.c3.get<int>(), W.o3q.get<int>(), t5 + 1));
+    OCTOK(cub_excl(W.ca.get<int>(), W.oaq.get<int>(), t5 + 1));
     int tot[3];
-    OCTOK(hipMemcpyAsync(&tot[0], W.o1.get<int>() + nb, 4, hipMemcpyDeviceToHost, s));
-    OCTOK(hipMemcpyAsync(&tot[1], W.o3.get<int>() + nb, 4, hipMemcpyDeviceToHost, s));
-    OCTOK(hipMemcpyAsync(&tot[2], W.oa.get<int>() + nb, 4, hipMemcpyDeviceToHost, s));
+    OCTOK(hipMemcpyAsync(&tot[0], W.o1q.get<int>() + t5, 4, hipMemcpyDeviceToHost, s));
+    OCTOK(hipMemcpyAsync(&tot[1], W.o3q.get<int>() + t5, 4, hipMemcpyDeviceToHost, s));
+    OCTOK(hipMemcpyAsync(&tot[2], W.oaq.get<int>() + t5, 4, hipMemcpyDeviceToHost, s));
     OCTOK(hipStreamSynchronize(s));
     const int n1own = tot[0], n3 = tot[1], nap = tot[2];
     OCTOK(W.t1.ensure((size_t)n1own * 4 + 4));
     for (OctreeWork::Impl::Buf *b : {&W.p4, &W.p4s}) OCTOK(b->ensure((size_t)n3 * 8 + 8));
     for (OctreeWork::Impl::Buf *b : {&W.pa, &W.pas}) OCTOK(b->ensure((size_t)nap * 8 + 8));
-    for (int t = 1; t <= 5; t++) OCTOK(W.st[t].ensure((size_t)(nb + 1) * 4));
     OCTOK(W.it[3].ensure((size_t)n3 * 4 + 4));
     OCTOK(W.it[4].ensure((size_t)n3 * 4 + 4));
-    hipLaunchKernelGGL(k_oct_desc, dim3(blocks(nb)), dim3(OCT_T), 0, s, B, nb, W.n5.get<int>(), W.l5.get<int>(), 1,
-                       nullptr, nullptr, nullptr, W.o1.get<int>(), W.o3.get<int>(), W.oa.get<int>(), W.t1.get<int>(),
-                       W.it[3].get<int>(), W.p4.get<uint64_t>(), W.pa.get<uint64_t>());
+    if (t5)
+        hipLaunchKernelGGL(k_oct_desc, dim3(blocks(t5)), dim3(OCT_T), 0, s, B, t5, W.owner.get<int>(),
+                           W.it[5].get<int>(), 1, nullptr, nullptr, nullptr, W.o1q.get<int>(), W.o3q.get<int>(),
+                           W.oaq.get<int>(), W.t1.get<int>(), W.it[3].get<int>(), W.p4.get<uint64_t>(),
+                           W.pa.get<uint64_t>());
+    // per-box starts of the own list-1 entries and of list 3
+    hipLaunchKernelGGL(k_oct_boxstart, dim3(blocks(nb + 1)), dim3(OCT_T), 0, s, nb, W.st[5].get<int>(),
+                       W.o1q.get<int>(), W.o1.get<int>());
+    hipLaunchKernelGGL(k_oct_boxstart, dim3(blocks(nb + 1)), dim3(OCT_T), 0, s, nb, W.st[5].get<int>(),
+                       W.o3q.get<int>(), W.st[3].get<int>());
     OCTOK(hipGetLastError());
     const int pbits = 32 + lbits;
-    // list 3: CSR as generated
-    OCTOK(hipMemcpyAsync(W.st[3].p, W.o3.p, (size_t)(nb + 1) * 4, hipMemcpyDeviceToDevice, s));
     // list 4: (j, b) pairs sorted, counted per j
     if (n3) OCTOK(cub_sortkeys64(W.p4.get<uint64_t>(), W.p4s.get<uint64_t>(), n3, pbits));
     OCTOK(hipMemsetAsync(W.cnt4.p, 0, (size_t)(nb + 1) * 4, s));
@@ -691,7 +754,7 @@ hipError_t build_tree_dev(const double *d_src, int n, const double *d_tgt, int n
                            W.acnt.get<int>());
     OCTOK(hipGetLastError());
     OCTOK(cub_excl(W.acnt.get<int>(), W.aexcl.get<int>(), nb + 1));
-    hipLaunchKernelGGL(k_oct_add, dim3(blocks(nb + 1)), dim3(OCT_T), 0, s, nb, W.acnt.get<int>(), W.c1.get<int>(),
+    hipLaunchKernelGGL(k_oct_cnt1, dim3(blocks(nb + 1)), dim3(OCT_T), 0, s, nb, W.acnt.get<int>(), W.o1.get<int>(),
                        W.cnt1.get<int>());
     OCTOK(hipGetLastError());
     OCTOK(cub_excl(W.cnt1.get<int>(), W.st[1].get<int>(), nb + 1));
@@ -699,24 +762,8 @@ hipError_t build_tree_dev(const double *d_src, int n, const double *d_tgt, int n
     if (nap)
         hipLaunchKernelGGL(k_oct_list1_app, dim3(blocks(nap)), dim3(OCT_T), 0, s, W.pas.get<uint64_t>(), nap,
                            W.st[1].get<int>(), W.aexcl.get<int>(), W.it[1].get<int>());
-    hipLaunchKernelGGL(k_oct_list1_own, dim3(blocks(nb)), dim3(OCT_T), 0, s, nb, W.c1.get<int>(), W.o1.get<int>(),
-                       W.acnt.get<int>(), W.t1.get<int>(), W.st[1].get<int>(), W.it[1].get<int>());
-    OCTOK(hipGetLastError());
-    // lists 5 and 2 from their tables (counts with a trailing zero)
-    OCTOK(hipMemsetAsync(W.n5.get<int>() + nb, 0, 4, s));
-    OCTOK(hipMemsetAsync(W.n2.get<int>() + nb, 0, 4, s));
-    OCTOK(cub_excl(W.n5.get<int>(), W.st[5].get<int>(), nb + 1));
-    OCTOK(cub_excl(W.n2.get<int>(), W.st[2].get<int>(), nb + 1));
-    int t5 = 0, t2 = 0;
-    OCTOK(hipMemcpyAsync(&t5, W.st[5].get<int>() + nb, 4, hipMemcpyDeviceToHost, s));
-    OCTOK(hipMemcpyAsync(&t2, W.st[2].get<int>() + nb, 4, hipMemcpyDeviceToHost, s));
-    OCTOK(hipStreamSynchronize(s));
-    OCTOK(W.it[5].ensure((size_t)t5 * 4 + 4));
-    OCTOK(W.it[2].ensure((size_t)t2 * 4 + 4));
-    hipLaunchKernelGGL(k_oct_rows, dim3(blocks(nb)), dim3(OCT_T), 0, s, nb, OCT_COLL, W.n5.get<int>(), W.l5.get<int>(),
-                       W.st[5].get<int>(), W.it[5].get<int>());
-    hipLaunchKernelGGL(k_oct_rows, dim3(blocks(nb)), dim3(OCT_T), 0, s, nb, OCT_L2, W.n2.get<int>(), W.l2.get<int>(),
-                       W.st[2].get<int>(), W.it[2].get<int>());
+    hipLaunchKernelGGL(k_oct_list1_own, dim3(blocks(nb)), dim3(OCT_T), 0, s, nb, W.o1.get<int>(), W.acnt.get<int>(),
+                       W.t1.get<int>(), W.st[1].get<int>(), W.it[1].get<int>());
     OCTOK(hipGetLastError());
 
     // to the host

@@ -100,7 +100,10 @@ def test_device_octree_matches_host(gpu_lib, host_tree_lib, case):
     print(f"octree case {name}: {len(src)} sources, {len(tgt)} targets, nbox {nbox}", flush=True)
     hs, h = host(host_tree_lib, src, tgt, nbox, imindlin)
     nb, ds, d = device_tree(gpu_lib, src, tgt, nbox, imindlin)
-    assert nb > 0 and np.array_equal(hs, ds), (nb, hs, ds)
+    assert nb > 0 and hs[0] == ds[0], (nb, hs, ds)
+    bad = np.argwhere(h[0] != d[0])
+    assert len(bad) == 0, (name, "boxes differ", bad[:5].tolist(), h[0][bad[0][0]].tolist(), d[0][bad[0][0]].tolist())
+    assert np.array_equal(hs, ds), (nb, hs, ds)
     for k, (a, b) in enumerate(zip(h, d)):
         assert np.array_equal(a, b), (name, ["boxes", "centers", "iz", "iztarg", "lstart", "litems"][k])
 

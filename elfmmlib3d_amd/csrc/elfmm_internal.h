@@ -40,7 +40,18 @@ struct Tree {
     std::vector<int> iz, iztarg;                   // sorted -> original
     // lists 1..5 in CSR, 0-based box ids; index [t] for t in 1..5
     std::vector<int> lstart[6], litems[6];
+    // side of each box when the boxes are not all of one root cube (a
+    // forest of octrees); empty: size / 2^level
+    std::vector<double> box_side;
 };
+
+// Forest of octrees: the trees of several point sets as one tree-like
+// structure for one plan. Boxes are renumbered level-major (every tree's
+// level-0 box, then every tree's level-1 boxes, ...), points keep their
+// trees' order with the sources (targets) of tree k starting at soff[k]
+// (toff[k]), and iz / iztarg map to the packed input order.
+void build_forest(const std::vector<const Tree *> &trees, const std::vector<int> &soff,
+                  const std::vector<int> &toff, Tree &F);
 
 // imindlin selects the half-space root rule of src/d3tstrcr_mindlin.f:1380.
 int build_tree(const double *src, int n, const double *targ, int nt, int nbox,

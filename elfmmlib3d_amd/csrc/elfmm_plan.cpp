@@ -152,7 +152,7 @@ void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool 
     P = Plan();
     for (double &v : P.stats) v = 0;
     P.sides.resize(nb);
-    for (int b = 0; b < nb; b++) P.sides[b] = T.size / (double)(1 << T.level[b]);
+    for (int b = 0; b < nb; b++) P.sides[b] = T.box_side.empty() ? T.size / (double)(1 << T.level[b]) : T.box_side[b];
     auto npts = [&](int b) { return (src_eval ? T.src_count[b] : 0) + T.tgt_count[b]; };
 
     if (ops) {
@@ -518,6 +518,89 @@ void build_mindlin_plan(const Tree &T, const Operators &ops, MindlinPlan &MP)
         k = e;
     }
     MP.m2l_tiles = g.tiles;
+}
+
+void build_forest(const std::vector<const Tree *> &trees, const std::vector<int> &soff, const std::vector<int> &toff,
+                  Tree &F)
+{
+    F = Tree();
+    int nt = (int)trees.size();
+    int L = 0;
+    for (const Tree *t : trees) L = std::max(L, t->nlev);
+    // new id of box b of tree k: level-major, trees in order within a level
+    std::vector<std::vector<int>> nid(nt);
+    std::vector<std::pair<int, int>> order;                 // (tree, box) by new id
+    F.laddr.clear();
+    for (int lev = 0; lev <= L; lev++) {
+        int first = (int)order.size();
+        for (int k = 0; k < nt; k++) {
+            const Tree &T = *trees[k];
+            if (nid[k].empty()) nid[k].assign(T.nboxes, -1);
+            if (lev > T.nlev) continue;
+            for (int b = T.laddr[2 * lev]; b < T.laddr[2 * lev] + T.laddr[2 * lev + 1]; b++) {
+                nid[k][b] = (int)order.size();
+                order.push_back({k, b});
+            }
+        }
+        F.laddr.push_back(first);
+        F.laddr.push_back((int)order.size() - first);
+    }
+    int nb = (int)order.size();
+    F.nboxes = nb;
+    F.nlev = L;
+    F.level.resize(nb);
+    F.ijk.resize(3 * (size_t)nb);
+    F.parent.resize(nb);
+    F.kids.assign(8 * (size_t)nb, -1);
+    F.nkids.resize(nb);
+    F.src_start.resize(nb);
+    F.src_count.resize(nb);
+    F.tgt_start.resize(nb);
+    F.tgt_count.resize(nb);
+    F.centers.resize(3 * (size_t)nb);
+    F.box_side.resize(nb);
+    for (int t = 1; t <= 5; t++) F.lstart[t].assign(nb + 1, 0);
+    for (int q = 0; q < nb; q++) {
+        int k = order[q].first, b = order[q].second;
+        const Tree &T = *trees[k];
+        F.level[q] = T.level[b];
+        for (int d = 0; d < 3; d++) {
+            F.ijk[3 * q + d] = T.ijk[3 * b + d];
+            F.centers[3 * q + d] = T.centers[3 * b + d];
+        }
+        F.parent[q] = T.parent[b] < 0 ? -1 : nid[k][T.parent[b]];
+        F.nkids[q] = T.nkids[b];
+        for (int c = 0; c < T.nkids[b]; c++) F.kids[8 * q + c] = nid[k][T.kids[8 * b + c]];
+        F.src_start[q] = soff[k] + T.src_start[b];
+        F.src_count[q] = T.src_count[b];
+        F.tgt_start[q] = toff[k] + T.tgt_start[b];
+        F.tgt_count[q] = T.tgt_count[b];
+        F.box_side[q] = T.size / (double)(1 << T.level[b]);
+        for (int t = 1; t <= 5; t++) F.lstart[t][q + 1] = F.lstart[t][q] + T.lstart[t][b + 1] - T.lstart[t][b];
+    }
+    for (int t = 1; t <= 5; t++) {
+        F.litems[t].resize(F.lstart[t][nb]);
+        parallel_chunks((nb + 255) / 256, [&](int ch, int) {
+            for (int q = ch * 256; q < std::min(nb, (ch + 1) * 256); q++) {
+                int k = order[q].first, b = order[q].second;
+                const Tree &T = *trees[k];
+                int o = F.lstart[t][q];
+                for (int e = T.lstart[t][b]; e < T.lstart[t][b + 1]; e++) F.litems[t][o++] = nid[k][T.litems[t][e]];
+            }
+        });
+    }
+    int ns = 0, ng = 0;
+    for (int k = 0; k < nt; k++) {
+        ns = std::max(ns, soff[k] + (int)trees[k]->iz.size());
+        ng = std::max(ng, toff[k] + (int)trees[k]->iztarg.size());
+    }
+    F.iz.resize(ns);
+    F.iztarg.resize(ng);
+    for (int k = 0; k < nt; k++) {
+        const Tree &T = *trees[k];
+        for (size_t i = 0; i < T.iz.size(); i++) F.iz[soff[k] + i] = soff[k] + T.iz[i];
+        for (size_t i = 0; i < T.iztarg.size(); i++) F.iztarg[toff[k] + i] = toff[k] + T.iztarg[i];
+    }
 }
 
 }  // namespace elfmm

@@ -234,11 +234,15 @@ int elfmm_gfx950_triatarg_dev(int iprec, double rlam, double rmu, int nparts,
  * caller's order: mesh m owns nparts[m] triangles (d_triangle 9 doubles,
  * d_trinorm, d_source, densities and d_ptfrc 3, d_strain 9 per triangle)
  * and ntargs[m] targets (ntargs may be NULL: no targets). The geometry is
- * copied; octrees, task lists and device buffers are built once. width in
- * 1..16 fixes the number of branches (concurrent chains of meshes) and
- * replays one graph built for them; -16..-1 fixes -width branches launched
- * directly on streams; 0 lets the tuner time widths 1, 2, 4, 8, 16 both
- * ways and keep the fastest. Returns ier
+ * copied; octrees, task lists and device buffers are built once. Two
+ * schedules: the forest (all meshes' octrees planned as one FMM whose task
+ * lists never cross meshes: one launch per phase for the whole batch) and
+ * branches (each mesh its own apply, meshes spread over concurrent chains).
+ * width in 1..16 fixes the number of branches and replays one graph built
+ * for them; -16..-1 fixes -width branches launched directly on streams;
+ * 100 / -100 the forest as a graph / direct launches; 0 lets the tuner time
+ * the forest and widths 1, 2, 4, 8, 16 both ways and keep the fastest
+ * (the buffers of the other schedule are released). Returns ier
  * (16 bad arguments, 4 tree failure, 8 host memory, 32 device error); on
  * success *batch is the handle. No counterpart in the reference, whose
  * callers loop over meshes. */
@@ -262,9 +266,9 @@ int elfmm_gfx950_batch_buffers(void *batch, double **d_sigma_sl, double **d_sigm
 int elfmm_gfx950_batch_apply(void *batch, const double *d_sigma_sl, const double *d_sigma_dl,
                              double *d_ptfrc, double *d_strain, double *d_ptfrctarg,
                              double *d_straintarg, void *stream);
-/* The schedule in use (branches, 1 = graph replay / 0 = direct launches)
- * and up to ntune (graph, width, milliseconds) triples of the schedules
- * the tuner timed; returns the number of schedules timed. */
+/* The schedule in use (branches, 0 for the forest; 1 = graph replay / 0 =
+ * direct launches) and up to ntune (graph, width, milliseconds) triples of
+ * the schedules the tuner timed; returns the number of schedules timed. */
 int elfmm_gfx950_batch_schedule(void *batch, int *width, int *graph, int ntune, double *tune);
 /* Releases the batch (waits for the device first). */
 int elfmm_gfx950_batch_destroy(void *batch);

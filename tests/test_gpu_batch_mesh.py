@@ -1,6 +1,7 @@
 """Batch of meshes (elfmm_gfx950_batch_*): one apply of many independent
-free-space meshes, as direct launches on branch streams or as the replay of
-a graph built for the batch, must give each mesh's elfmm3dtriatarg. Meshes of uneven sizes (icospheres of 20
+free-space meshes, as one FMM over the forest of their octrees or as one
+apply per mesh over branch streams, launched directly or as the replay of a
+graph built for the batch, must give each mesh's elfmm3dtriatarg. Meshes of uneven sizes (icospheres of 20
 to ~3,000 triangles, some with targets and some without) cross the tuner's
 schedules; each graph replay is checked against the single-mesh applies,
 with other device work (single-mesh applies, torch kernels) between the
@@ -73,11 +74,13 @@ def case(gpu_lib):
     return ws, dens, ref
 
 
-def test_direct_launches_match_single_meshes(gpu_lib, case):
+@pytest.mark.parametrize("width", [-3, -100])
+def test_direct_launches_match_single_meshes(gpu_lib, case, width):
+    """Branches (-3) and the forest (-100: all meshes' octrees as one FMM)."""
     import torch
     ws, dens, ref = case
-    B = gpu_lib.MeshBatch(2, 1.0, 1.0, torch_meshes(ws), width=-3)
-    assert B.schedule()[:2] == (3, 0)
+    B = gpu_lib.MeshBatch(2, 1.0, 1.0, torch_meshes(ws), width=width)
+    assert B.schedule()[:2] == (3 if width == -3 else 0, 0)
     T = lambda a: torch.as_tensor(a, device="cuda")
     for d in (0, 1):
         outs = B.apply(T(dens[d][0]), T(dens[d][1]))
@@ -86,13 +89,13 @@ def test_direct_launches_match_single_meshes(gpu_lib, case):
     B.close()
 
 
-@pytest.mark.parametrize("width", [1, 4])
+@pytest.mark.parametrize("width", [1, 4, 100])
 def test_graph_replays_between_other_work(gpu_lib, case, width):
     import torch
     ws, dens, ref = case
     B = gpu_lib.MeshBatch(2, 1.0, 1.0, torch_meshes(ws), width=width)
     w, graph, _ = B.schedule()
-    assert (w, graph) == (min(width, len(ws)), 1)
+    assert (w, graph) == (0 if width == 100 else min(width, len(ws)), 1)
     T = lambda a: torch.as_tensor(a, device="cuda")
     a = torch.randn(512, 512, dtype=torch.float64, device="cuda")
     for rep in range(4):
@@ -113,8 +116,8 @@ def test_tuned_batch_matches_single_meshes(gpu_lib, case, iprec):
     ws, dens, ref = case
     B = gpu_lib.MeshBatch(iprec, 1.0, 1.0, torch_meshes(ws), width=0)
     width, graph, tune = B.schedule()
-    assert width in (1, 2, 4, 8, 16) and graph in (0, 1)
-    assert len(tune) == 2 * len([w for w in (1, 2, 4, 8, 16) if w == 1 or w <= len(ws)])
+    assert width in (0, 1, 2, 4, 8, 16) and graph in (0, 1)
+    assert len(tune) == 2 * (1 + len([w for w in (1, 2, 4, 8, 16) if w == 1 or w <= len(ws)]))
     T = lambda a: torch.as_tensor(a, device="cuda")
     for d in (0, 1, 0):
         outs = B.apply(T(dens[d][0]), T(dens[d][1]))

@@ -350,6 +350,30 @@ def c4_batched(args):
             "batch_vs_plain_rel": same, "disp_err_vs_direct": err, "checked_meshes": pick}
 
 
+def c4_forest(args):
+    """The C4 meshes of c4_batched under the forest schedule only (direct
+    launches): one warm-up and three timed applies, for kernel profiles."""
+    import torch
+    import elfmmlib3d_amd as E
+    from elfmmlib3d_amd import mesh
+    ws = mesh.sphere_batch(args.c4_meshes, 500, 20000, 2025)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+    meshes = [(T(w[0]), T(w[1]), T(w[2]), None) for w in ws]
+    ssl = T(np.concatenate([w[3] for w in ws]))
+    sdl = T(np.concatenate([w[4] for w in ws]))
+    B = E.MeshBatch(args.iprec, 1.0, 1.0, meshes, ifptfrctarg=False, ifstraintarg=False, width=-100)
+    B.apply(ssl, sdl)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        B.apply(ssl, sdl)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / 3 * 1e3
+    B.close()
+    return {"workload": f"C4 meshes as one forest: {len(ws)} icospheres, {sum(w[0].shape[0] for w in ws)} triangles",
+            "apply_ms": ms}
+
+
 def c5_served(args):
     """Config C5 on one GPU: a Poisson stream of requests at 0.5x, 1x and 2x
     the measured capacity; 60% half-space fault meshes (8 faults, targets =
@@ -490,6 +514,8 @@ def main():
     ap.add_argument("--no-c4", dest="c4", action="store_false", help="skip the C4 many-mesh measurement")
     ap.add_argument("--c4-meshes", type=int, default=256, help="meshes of the C4 batch")
     ap.add_argument("--c4-only", action="store_true", help="run only the C4 many-mesh measurement")
+    ap.add_argument("--c4-forest", action="store_true",
+                    help="C4 meshes as the forest schedule only, three applies (for profiling)")
     ap.add_argument("--c5-max", type=int, default=500000, help="largest triangle count of a C5 request")
     args = ap.parse_args()
     if args.kernel_pass:
@@ -500,6 +526,9 @@ def main():
         return
     if args.c4_only:
         print(json.dumps(c4_batched(args)), flush=True)
+        return
+    if args.c4_forest:
+        print(json.dumps(c4_forest(args)), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "the library runs on one GPU per process; --gpus must be 1"}))

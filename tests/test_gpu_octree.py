@@ -25,7 +25,7 @@ def host_tree_lib(tmp_path_factory):
     out = str(tmp_path_factory.mktemp("tree") / "libtree_check.so")
     subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-o", out,
                     os.path.join(ROOT, "tests", "csrc", "tree_check.cpp"),
-                    os.path.join(CSRC, "elfmm_tree.cpp"), os.path.join(CSRC, "elfmm_plan.cpp"),
+                    os.path.join(CSRC, "elfmm_tree.cpp"), os.path.join(CSRC, "elfmm_plan.cpp"), os.path.join(CSRC, "elfmm_ops.cpp"),
                     os.path.join(CSRC, "elfmm_pool.cpp")], check=True)
     return C.CDLL(out)
 

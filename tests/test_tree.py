@@ -22,7 +22,7 @@ def tree_lib(tmp_path_factory):
     out = str(tmp_path_factory.mktemp("tree") / "libtree_check.so")
     subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-o", out,
                     os.path.join(ROOT, "tests", "csrc", "tree_check.cpp"),
-                    os.path.join(CSRC, "elfmm_tree.cpp"), os.path.join(CSRC, "elfmm_plan.cpp"),
+                    os.path.join(CSRC, "elfmm_tree.cpp"), os.path.join(CSRC, "elfmm_plan.cpp"), os.path.join(CSRC, "elfmm_ops.cpp"),
                     os.path.join(CSRC, "elfmm_pool.cpp")],
                    check=True)
     return C.CDLL(out)
@@ -109,3 +109,23 @@ def test_iter_near_lists_match_reference_store_size(tree_lib, case, nbox):
     tree_lib.plan_iter_ref_words.restype = C.c_double
     words = tree_lib.plan_iter_ref_words(len(src), 1, 1, 1, 1, 1, 1)
     assert int(words) == int(g["lused"])
+
+
+def test_forest_matches_trees(tree_lib):
+    """The forest of several trees (one plan for many meshes): the same boxes
+    and lists as the trees, remapped, and the plan's work the sum of the
+    trees' plans (near pairs, M2L pairs, P2M boxes, P2L and M2T entries)."""
+    rng = np.random.default_rng(4)
+    pts, cnt = [], []
+    for k, ns in enumerate([3, 7, 12, 1, 9]):
+        c = mesh.centroids(mesh.icosphere(ns)) * rng.uniform(0.5, 3) + rng.uniform(-5, 5, 3)
+        pts.append(c)
+        cnt.append(len(c))
+    src = np.ascontiguousarray(np.concatenate(pts))
+    out = np.zeros(13)
+    counts = np.array(cnt, dtype=np.int32)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    assert tree_lib.forest_check(p(src), p(counts), len(cnt), 20, 8, p(out)) == 0
+    for i in range(0, 12, 2):
+        assert out[i] == out[i + 1], (i, out)
+    assert out[2] > 0 and out[4] > 0 and out[12] == 1, out

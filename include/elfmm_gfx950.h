@@ -297,8 +297,10 @@ void elfmm_gfx950_octree_get(int *boxes, double *centers, int *iz, int *iztarg, 
  * [6] L2L, [7] L2T, [8] M2T, [9] near field, [10] assembly, [11] total
  * wall time of the call, [12] octree and lists (part of [0]), [13]
  * interaction plan (part of [0]); after a half-space call [0]-[13] are
- * those of its anti-image and image near-field pass, [14] the wall time of
- * its Mindlin far field and [15] the wall time of the whole call. [0],
+ * those of its anti-image and image near-field pass, [14] the time of its
+ * Mindlin far field (wall time after the anti-image pass; in a replay of
+ * one set, where it runs beside that pass, its device time) and [15] the
+ * wall time of the whole call. [0],
  * [12], [13] are host times; [1]-[10] are HIP-event times on the stream
  * each phase runs on (the near field overlaps [2]-[8]). */
 void elfmm_gfx950_last_timings(double *ms16);

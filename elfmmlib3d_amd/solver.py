@@ -3,9 +3,10 @@
 gmres_simple follows the reference's matlab/gmres_simple.m: a GMRES in
 GCR form that keeps the orthonormalised images A e_i and the directions
 e_i, re-orthogonalises once, recomputes the residual every 10 steps and
-stops with ier 8 when the residual stops decreasing, ier 4 at maxit. It
-works on any vectors that support the torch/numpy operations used (torch
-tensors on the GPU keep the whole solve in HBM).
+stops with ier 8 when the residual stops decreasing, ier 4 at maxit.
+bicgstab_simple follows matlab/bicgstab_simple.m. Both work on any vectors
+that support the torch/numpy operations used (torch tensors on the GPU keep
+the whole solve in HBM).
 
 IterOperator is the matrix-vector product of the elastic layer potentials
 at the triangle centroids through elfmm3dtriatargiter: the first product
@@ -75,6 +76,78 @@ def gmres_simple(A, b, tol=1e-6, maxit=None, x0=None, log=None):
         if log:
             log(f"iter: {i}, rel=norm(r,2)/norm(b,2): {er1 / nb:17.12f}")
         if er1 < tol * nb:
+            return x, 0, hist
+    return x, 4, hist
+
+
+def _sqrt(v):
+    """Square root of a scalar as MATLAB takes it: real for a real v >= 0,
+    complex otherwise."""
+    import cmath
+    import math
+    v = complex(v) if not isinstance(v, (float, int)) else v
+    if isinstance(v, complex) and v.imag == 0.0:
+        v = v.real
+    if isinstance(v, complex) or v < 0:
+        return cmath.sqrt(v)
+    return math.sqrt(v)
+
+
+def _scalar(v):
+    v = v.item() if hasattr(v, "item") else v
+    return v.real if isinstance(v, complex) and v.imag == 0.0 else v
+
+
+def bicgstab_simple(A, b, tol=1e-6, maxit=None, log=None):
+    """Solve A x = b by the reference's BiCGSTAB (matlab/bicgstab_simple.m:
+    36-108). Returns (x, ier, history); history holds norm(r)/norm(b) after
+    each step.
+
+    The recurrence is the reference's: x0 = 0, the shadow residual fixed to
+    the first residual, two products per step (steps i = 2..maxit, so
+    maxit - 1 of them), and beta = (sqrt(rho_i) / sqrt(rho_{i-1})) *
+    (alpha / omega) with rho = <r0, r>, the ratio of square roots the
+    listing takes (:88-90); it stops when norm(r) < tol * norm(r0)
+    (:58,101-102; norm(r0) = norm(b) for a linear A). Two differences, both in
+    what the listing returns rather than in the steps: ier is 0 when the
+    tolerance is met and 4 when maxit is reached (the header's meaning;
+    the listing sets ier = 4 after the loop either way, :108), and a
+    1 x 1 system is solved as b / A(1) (the listing divides b by A(0),
+    :42-45, which is 0 for a linear A)."""
+    if not callable(A):
+        M = A
+        A = (lambda x: M @ x)
+    n = b.shape[0]
+    if maxit is None:
+        maxit = min(n, 20)
+    if n == 1:
+        one = b * 0 + 1
+        return b / A(one), 0, []
+    x = b * 0
+    r = b - A(x)
+    p = r
+    r0 = r
+    norm1 = _norm(r)
+    nb = _norm(b)
+    er1 = _sqrt(_scalar(_dot(r0, r)))
+    hist = []
+    for i in range(2, maxit + 1):
+        v = A(p)
+        alpha = _scalar(_dot(r0, r)) / _scalar(_dot(r0, v))
+        s = r - alpha * v
+        t = A(s)
+        omega = _scalar(_dot(t, s)) / _scalar(_dot(t, t))
+        x = x + alpha * p + omega * s
+        r = s - omega * t
+        er = _norm(r)
+        er2 = _sqrt(_scalar(_dot(r0, r)))
+        beta = (er2 / er1) * (alpha / omega)
+        er1 = er2
+        p = r + beta * (p - omega * v)
+        hist.append(er / nb)
+        if log:
+            log(f"iter: {i}, rel=norm(r,2)/norm(b,2): {er / nb:17.12f}")
+        if er < tol * norm1:
             return x, 0, hist
     return x, 4, hist
 

@@ -109,3 +109,37 @@ def test_iter_operator_on_device(gpu_lib):
     # reference's icomp 2/3 form), equal to the store call's result
     assert rel(y2, y) < 1e-10
     assert op.release() == 2
+
+
+def test_bicgstab_solves_nonsymmetric_system():
+    """Converges to the dense solution; one step on A = c I is exact
+    (alpha = 1/c, the residual vanishes)."""
+    from elfmmlib3d_amd.solver import bicgstab_simple
+    rng = np.random.default_rng(3)
+    n = 50
+    A = np.eye(n) * 4 + rng.standard_normal((n, n)) / np.sqrt(n)
+    x_true = rng.standard_normal(n)
+    b = A @ x_true
+    x, ier, hist = bicgstab_simple(A, b, tol=1e-12, maxit=60)
+    assert ier == 0
+    assert np.linalg.norm(x - x_true) / np.linalg.norm(x_true) < 1e-10
+    assert np.linalg.norm(A @ x - b) < 1e-12 * np.linalg.norm(b)
+    x, ier, hist = bicgstab_simple(2.5 * np.eye(7), b[:7], tol=1e-12, maxit=5)
+    assert ier == 0 and len(hist) == 1 and np.allclose(x, b[:7] / 2.5, rtol=0, atol=1e-15)
+    x, ier, hist = bicgstab_simple(A, b, tol=1e-14, maxit=3)
+    assert ier == 4 and len(hist) == 2
+    x, ier, _ = bicgstab_simple(np.array([[4.0]]), np.array([2.0]))
+    assert ier == 0 and x[0] == 0.5
+
+
+def test_bicgstab_torch_tensors():
+    torch = pytest.importorskip("torch")
+    from elfmmlib3d_amd.solver import bicgstab_simple
+    g = torch.Generator().manual_seed(4)
+    n = 30
+    # complex, as the listing's header describes: its beta (a ratio of
+    # square roots of <r0, r>) turns a real system complex when <r0, r> < 0
+    A = torch.eye(n, dtype=torch.complex128) * 3 + torch.randn(n, n, generator=g, dtype=torch.complex128) / 6
+    b = torch.randn(n, generator=g, dtype=torch.complex128)
+    x, ier, _ = bicgstab_simple(lambda v: A @ v, b, tol=1e-12, maxit=30)
+    assert ier == 0 and float(torch.linalg.norm(A @ x - b) / torch.linalg.norm(b)) < 1e-11

@@ -154,6 +154,12 @@ constexpr int EW_PTS = 32;                           // points per evaluation wa
 // field (src/elfmm3dtria_iter_1.f:1061-1110).
 void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool direct_only, int nrhs,
                 Plan &P, bool iter = false);
+// The two halves of build_plan, for a caller that starts the near field
+// before the far-field lists exist: build_plan_near fills the near-field
+// lists (nf_*, stats[6]) of P, build_plan_far everything else; neither
+// clears what the other fills (P starts as Plan() with zero stats).
+void build_plan_near(const Tree &T, bool src_eval, bool direct_only, bool iter, Plan &P);
+void build_plan_far(const Tree &T, int n, const Operators *ops, bool src_eval, int nrhs, Plan &P, bool iter);
 
 // Task lists of the Mindlin harmonic far field (lfmm3d_list2_mindlin,
 // src/lfmm3droutsmindlin.f:28-529) on its own octree, beside the Laplace

@@ -142,15 +142,91 @@ void wave_tasks(const std::vector<I> &off, std::vector<int> &wt)
 
 }  // namespace
 
+void build_plan_near(const Tree &T, bool src_eval, bool direct_only, bool iter, Plan &P)
+{
+    int nb = T.nboxes;
+    auto npts = [&](int b) { return (src_eval ? T.src_count[b] : 0) + T.tgt_count[b]; };
+    P.nf_boxes.clear();
+    P.nf_start.clear();
+    P.nf_items.clear();
+    // leaves: near field over self + list 1. With iter, lists 3 and 4 join
+    // the near field; the reference evaluates a box's list 4 at every point
+    // below it, so a leaf also takes the list 4 of each of its ancestors and
+    // every point is handled once, by its leaf.
+    P.nf_start.push_back(0);
+    for (int b = 0; b < nb; b++) {
+        if (T.nkids[b] || !npts(b)) continue;
+        size_t before = P.nf_items.size();
+        for (int q = T.lstart[1][b]; q < T.lstart[1][b + 1]; q++) {
+            int j = T.litems[1][q];
+            if (T.src_count[j]) P.nf_items.push_back(j);
+        }
+        if (T.src_count[b]) P.nf_items.push_back(b);
+        if (iter) {
+            for (int q = T.lstart[3][b]; q < T.lstart[3][b + 1]; q++) {
+                int j = T.litems[3][q];
+                if (T.src_count[j]) P.nf_items.push_back(j);
+            }
+            for (int a = b; a >= 0; a = T.parent[a])
+                for (int q = T.lstart[4][a]; q < T.lstart[4][a + 1]; q++) {
+                    int j = T.litems[4][q];
+                    if (T.src_count[j]) P.nf_items.push_back(j);
+                }
+        }
+        if (P.nf_items.size() == before) continue;
+        P.nf_boxes.push_back(b);
+        P.nf_start.push_back((int)P.nf_items.size());
+    }
+    {
+        // one workgroup per box, heaviest (points x triangles) first, so
+        // the long boxes do not start last
+        size_t nq = P.nf_boxes.size();
+        std::vector<double> w(nq);
+        std::vector<int> ord(nq);
+        for (size_t q = 0; q < nq; q++) {
+            double ns = 0;
+            for (int k = P.nf_start[q]; k < P.nf_start[q + 1]; k++) ns += T.src_count[P.nf_items[k]];
+            w[q] = ns * npts(P.nf_boxes[q]);
+            ord[q] = (int)q;
+        }
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return w[a] > w[b]; });
+        std::vector<int> bx(nq), st(1, 0), it;
+        it.reserve(P.nf_items.size());
+        for (size_t q = 0; q < nq; q++) {
+            int o = ord[q];
+            bx[q] = P.nf_boxes[o];
+            it.insert(it.end(), P.nf_items.begin() + P.nf_start[o], P.nf_items.begin() + P.nf_start[o + 1]);
+            st.push_back((int)it.size());
+        }
+        P.nf_boxes.swap(bx);
+        P.nf_start.swap(st);
+        P.nf_items.swap(it);
+    }
+    if (direct_only) P.nf_boxes.clear();
+    double npair_near = 0;
+    for (size_t q = 0; q < P.nf_boxes.size(); q++) {
+        double ns = 0;
+        for (int k = P.nf_start[q]; k < P.nf_start[q + 1]; k++) ns += T.src_count[P.nf_items[k]];
+        npair_near += npts(P.nf_boxes[q]) * ns;
+    }
+    P.stats[6] = npair_near;
+}
+
 void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool direct_only, int nrhs,
                 Plan &P, bool iter)
+{
+    P = Plan();
+    for (double &v : P.stats) v = 0;
+    build_plan_near(T, src_eval, direct_only, iter, P);
+    build_plan_far(T, n, ops, src_eval, nrhs, P, iter);
+}
+
+void build_plan_far(const Tree &T, int n, const Operators *ops, bool src_eval, int nrhs, Plan &P, bool iter)
 {
     int nb = T.nboxes;
     int p = ops ? ops->p : 0;
     int nc = ncoef(p);
     int nchunk = (nb + CHUNK - 1) / CHUNK;
-    P = Plan();
-    for (double &v : P.stats) v = 0;
     P.sides.resize(nb);
     for (int b = 0; b < nb; b++) P.sides[b] = T.box_side.empty() ? T.size / (double)(1 << T.level[b]) : T.box_side[b];
     auto npts = [&](int b) { return (src_eval ? T.src_count[b] : 0) + T.tgt_count[b]; };
@@ -310,61 +386,9 @@ void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool 
         P.p2l_items.swap(it);
     }
 
-    // leaves: L2T (level >= 2) and near field over self + list 1. With
-    // iter, lists 3 and 4 join the near field; the reference evaluates a
-    // box's list 4 at every point below it, so a leaf also takes the list 4
-    // of each of its ancestors and every point is handled once, by its leaf.
-    P.nf_start.push_back(0);
-    for (int b = 0; b < nb; b++) {
-        if (T.nkids[b] || !npts(b)) continue;
-        if (ops && T.level[b] >= 2) P.l2t_boxes.push_back(b);
-        size_t before = P.nf_items.size();
-        for (int q = T.lstart[1][b]; q < T.lstart[1][b + 1]; q++) {
-            int j = T.litems[1][q];
-            if (T.src_count[j]) P.nf_items.push_back(j);
-        }
-        if (T.src_count[b]) P.nf_items.push_back(b);
-        if (iter) {
-            for (int q = T.lstart[3][b]; q < T.lstart[3][b + 1]; q++) {
-                int j = T.litems[3][q];
-                if (T.src_count[j]) P.nf_items.push_back(j);
-            }
-            for (int a = b; a >= 0; a = T.parent[a])
-                for (int q = T.lstart[4][a]; q < T.lstart[4][a + 1]; q++) {
-                    int j = T.litems[4][q];
-                    if (T.src_count[j]) P.nf_items.push_back(j);
-                }
-        }
-        if (P.nf_items.size() == before) continue;
-        P.nf_boxes.push_back(b);
-        P.nf_start.push_back((int)P.nf_items.size());
-    }
-    {
-        // one workgroup per box, heaviest (points x triangles) first, so
-        // the long boxes do not start last
-        size_t nq = P.nf_boxes.size();
-        std::vector<double> w(nq);
-        std::vector<int> ord(nq);
-        for (size_t q = 0; q < nq; q++) {
-            double ns = 0;
-            for (int k = P.nf_start[q]; k < P.nf_start[q + 1]; k++) ns += T.src_count[P.nf_items[k]];
-            w[q] = ns * npts(P.nf_boxes[q]);
-            ord[q] = (int)q;
-        }
-        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return w[a] > w[b]; });
-        std::vector<int> bx(nq), st(1, 0), it;
-        it.reserve(P.nf_items.size());
-        for (size_t q = 0; q < nq; q++) {
-            int o = ord[q];
-            bx[q] = P.nf_boxes[o];
-            it.insert(it.end(), P.nf_items.begin() + P.nf_start[o], P.nf_items.begin() + P.nf_start[o + 1]);
-            st.push_back((int)it.size());
-        }
-        P.nf_boxes.swap(bx);
-        P.nf_start.swap(st);
-        P.nf_items.swap(it);
-    }
-    if (direct_only) P.nf_boxes.clear();
+    // L2T: the leaves with points at level >= 2
+    for (int b = 0; b < nb; b++)
+        if (!T.nkids[b] && npts(b) && ops && T.level[b] >= 2) P.l2t_boxes.push_back(b);
 
     // evaluation tasks: L2T one per point of each leaf; M2T one per (point,
     // list-3 box), grouped by the box whose field operand is read
@@ -433,13 +457,6 @@ void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool 
     P.stats[0] = nb;
     P.stats[1] = T.nlev;
     P.stats[2] = p;
-    double npair_near = 0;
-    for (size_t q = 0; q < P.nf_boxes.size(); q++) {
-        double ns = 0;
-        for (int k = P.nf_start[q]; k < P.nf_start[q + 1]; k++) ns += T.src_count[P.nf_items[k]];
-        npair_near += npts(P.nf_boxes[q]) * ns;
-    }
-    P.stats[6] = npair_near;
     P.stats[7] = (double)std::count_if(P.m2l_tiles.begin(), P.m2l_tiles.end(), [](const M2LTile &t) { return t.npair > 0; });
     double m2m_pairs = 0, l2l_pairs = 0;
     for (auto &g : P.m2m_l) m2m_pairs += g.src.size();

@@ -196,20 +196,33 @@ void build_operators(int p, const int *itable, Operators &ops)
     // parts and the imaginary parts to imaginary parts, each by a real
     // ncoef(pt)/2 square matrix (rows j,k; columns l,m >= 0). Stored per
     // offset: the real-part matrix, then the imaginary-part matrix.
+    // The rotated operator depends on the offset only through rho^2 =
+    // i^2 + j^2, k and pt: offsets that share them (up to eight, mirror
+    // images in x, y and the diagonal) share one stored copy.
     size_t tot = 0;
+    std::vector<int> owner(343, -1);
     for (int o = 0; o < 343; o++) {
         int i = o / 49 - 3, j = (o / 7) % 7 - 3, k = o % 7 - 3;
         int far = std::max(std::abs(i), std::max(std::abs(j), std::abs(k))) >= 2;
         int pt = far ? std::min(p, itable[o]) : 0;
         ops.m2l_pt[o] = far ? pt : -1;
-        ops.m2l_off[o] = (int64_t)tot;
         ops.m2l_phi[o] = far ? std::atan2(-(double)j, -(double)i) : 0.0;
-        if (far) tot += (size_t)2 * (ncoef(pt) / 2) * (ncoef(pt) / 2);
+        ops.m2l_off[o] = (int64_t)tot;
+        if (!far) continue;
+        for (int q = 0; q < o; q++) {
+            int iq = q / 49 - 3, jq = (q / 7) % 7 - 3, kq = q % 7 - 3;
+            if (ops.m2l_pt[q] == pt && kq == k && iq * iq + jq * jq == i * i + j * j) {
+                owner[o] = q;
+                ops.m2l_off[o] = ops.m2l_off[q];
+                break;
+            }
+        }
+        if (owner[o] < 0) tot += (size_t)2 * (ncoef(pt) / 2) * (ncoef(pt) / 2);
     }
     ops.m2l.assign(tot, 0.0);
     for (int o = 0; o < 343; o++) {
         int pt = ops.m2l_pt[o];
-        if (pt < 0) continue;
+        if (pt < 0 || owner[o] >= 0) continue;
         int i = o / 49 - 3, j = (o / 7) % 7 - 3, k = o % 7 - 3;
         int n = ncoef(pt), h = n / 2;
         double rho = std::sqrt((double)(i * i + j * j));

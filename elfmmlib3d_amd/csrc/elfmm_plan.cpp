@@ -304,18 +304,27 @@ void build_plan_far(const Tree &T, int n, const Operators *ops, bool src_eval, i
         }
         double dense = 0, alg = 0;
         // the pairs are already in offset order; tiles per offset cover
-        // ML_P pairs by ML_M coefficients of that offset's rotated operator
-        std::vector<int64_t> tstart(344, 0);
-        for (int o = 0; o < 343; o++) {
+        // ML_P pairs by ML_M coefficients of that offset's rotated operator.
+        // Offsets sharing an operator lay out their tiles together (tile
+        // slot order: offsets by operator), so that xcd_order keeps them in
+        // one run.
+        std::vector<int> oord(343);
+        for (int o = 0; o < 343; o++) oord[o] = o;
+        std::stable_sort(oord.begin(), oord.end(), [&](int a, int b) { return ops->m2l_off[a] < ops->m2l_off[b]; });
+        std::vector<int64_t> tstart(343, 0);
+        int64_t ntile = 0;
+        for (int q = 0; q < 343; q++) {
+            int o = oord[q];
             int m = ostart[o + 1] - ostart[o];
             int h = ncoef(ops->m2l_pt[o]) / 2;
-            tstart[o + 1] = tstart[o] + (m ? (int64_t)((m + ML_P - 1) / ML_P) * ((h + ML_M - 1) / ML_M) : 0);
+            tstart[o] = ntile;
+            ntile += m ? (int64_t)((m + ML_P - 1) / ML_P) * ((h + ML_M - 1) / ML_M) : 0;
             if (!m) continue;
             // two real h x h products per pair and right-hand side
             dense += m * 2.0 * 2.0 * h * (double)h * nrhs;
             alg += m * alg_m2l[ops->m2l_pt[o]] * nrhs;
         }
-        P.m2l_tiles.resize(tstart[343]);
+        P.m2l_tiles.resize(ntile);
         parallel_chunks(343, [&](int o, int) {
             int m = ostart[o + 1] - ostart[o];
             if (!m) return;

@@ -17,6 +17,10 @@ from ._lib import (  # noqa: F401
     elhfmm3dtriatargiter,
     half_parts,
     elfmm3dtria,
+    el3dpartdirecttarg,
+    elfmm3dpart,
+    elfmm3dparttarg,
+    elfmm3dparttarg_dev,
     elfmm3dtriatarg,
     elfmm3dtriatarg_dev,
     elfmm3dtriatargiter,
@@ -32,7 +36,8 @@ from ._lib import (  # noqa: F401
 )
 from ._build import build  # noqa: F401
 
-__all__ = ["elfmm3dtria", "elfmm3dtriatarg", "el3dtriadirecttarg", "elfmm3dtriatarg_dev",
+__all__ = ["elfmm3dpart", "elfmm3dparttarg", "el3dpartdirecttarg", "elfmm3dparttarg_dev",
+           "elfmm3dtria", "elfmm3dtriatarg", "el3dtriadirecttarg", "elfmm3dtriatarg_dev",
            "elfmm3dtriatargiter", "elhfmm3dtriatargiter", "elh3dtriadirecttarg", "half_parts",
            "iter_release",
            "last_stats", "last_timings", "library", "library_path", "build", "elfmm3dtriatargiter_dev", "elhfmm3dtriatargiter_dev", "expansion_order", "triatargiter_multi_dev",

@@ -34,6 +34,7 @@ def library():
         _lib.elfmm_gfx950_triatarg_dev.restype = C.c_int
         _lib.elfmm_gfx950_triatargiter_dev.restype = C.c_int
         _lib.elfmm_gfx950_htriatargiter_dev.restype = C.c_int
+        _lib.elfmm_gfx950_parttarg_dev.restype = C.c_int
         _lib.elfmm_gfx950_version.restype = C.c_char_p
     return _lib
 
@@ -202,6 +203,81 @@ def iter_release(wsave):
     """Free the device-resident interactions named by wsave. Returns 1 if
     a store was freed."""
     return library().elfmm_gfx950_iter_release(_p(wsave), C.c_int(wsave.size))
+
+
+def _prep_part(source, sigma_sl, sigma_dl, sigma_dv, target):
+    n = source.shape[0]
+    src = _f64(source, (n, 3))
+    ifs = int(sigma_sl is not None)
+    ifd = int(sigma_dl is not None)
+    if ifd and sigma_dv is None:
+        raise ValueError("sigma_dl needs sigma_dv (dipole orientations)")
+    ssl = _f64(sigma_sl if ifs else np.zeros((n, 3)), (n, 3))
+    sdl = _f64(sigma_dl if ifd else np.zeros((n, 3)), (n, 3))
+    sdv = _f64(sigma_dv if ifd else np.zeros((n, 3)), (n, 3))
+    if target is None:
+        target = np.zeros((0, 3))
+    m = target.shape[0]
+    tgt = _f64(target if m else np.zeros((1, 3)), (max(m, 1), 3))
+    return n, src, ifs, ssl, ifd, sdl, sdv, m, tgt
+
+
+def elfmm3dparttarg(iprec, rlam, rmu, source, sigma_sl=None, sigma_dl=None, sigma_dv=None, target=None,
+                    ifptfrc=1, ifstrain=1, ifptfrctarg=1, ifstraintarg=1):
+    """FMM displacement and strain of point sources at the sources (self
+    pair excluded) and at targets (src/elfmm3dpart.f:104). Returns
+    (ier, ptfrc, strain, ptfrctarg, straintarg)."""
+    n, src, ifs, ssl, ifd, sdl, sdv, m, tgt = _prep_part(source, sigma_sl, sigma_dl, sigma_dv, target)
+    pt, st = np.zeros((n, 3)), np.zeros((n, 3, 3))
+    ptt, stt = np.zeros((max(m, 1), 3)), np.zeros((max(m, 1), 3, 3))
+    ier = C.c_int(0)
+    library().elfmm3dparttarg_(C.byref(ier), _iref(iprec), _dref(rlam), _dref(rmu), _iref(n), _p(src),
+                               _iref(ifs), _p(ssl), _iref(ifd), _p(sdl), _p(sdv), _iref(ifptfrc), _p(pt),
+                               _iref(ifstrain), _p(st), _iref(m), _p(tgt), _iref(ifptfrctarg if m else 0),
+                               _p(ptt), _iref(ifstraintarg if m else 0), _p(stt))
+    return ier.value, pt, st, ptt[:m], stt[:m]
+
+
+def elfmm3dpart(iprec, rlam, rmu, source, sigma_sl=None, sigma_dl=None, sigma_dv=None, ifptfrc=1, ifstrain=1):
+    """elfmm3dparttarg without targets (src/elfmm3dpart.f:52). Returns
+    (ier, ptfrc, strain)."""
+    n, src, ifs, ssl, ifd, sdl, sdv, _, _ = _prep_part(source, sigma_sl, sigma_dl, sigma_dv, None)
+    pt, st = np.zeros((n, 3)), np.zeros((n, 3, 3))
+    ier = C.c_int(0)
+    library().elfmm3dpart_(C.byref(ier), _iref(iprec), _dref(rlam), _dref(rmu), _iref(n), _p(src), _iref(ifs),
+                           _p(ssl), _iref(ifd), _p(sdl), _p(sdv), _iref(ifptfrc), _p(pt), _iref(ifstrain), _p(st))
+    return ier.value, pt, st
+
+
+def el3dpartdirecttarg(rlam, rmu, source, sigma_sl=None, sigma_dl=None, sigma_dv=None, target=None,
+                       ifptfrc=1, ifstrain=1, ifptfrctarg=1, ifstraintarg=1):
+    """O(N^2) point-source sum without self pairs (src/elfmm3dpart.f:506).
+    Returns (ptfrc, strain, ptfrctarg, straintarg)."""
+    n, src, ifs, ssl, ifd, sdl, sdv, m, tgt = _prep_part(source, sigma_sl, sigma_dl, sigma_dv, target)
+    pt, st = np.zeros((n, 3)), np.zeros((n, 3, 3))
+    ptt, stt = np.zeros((max(m, 1), 3)), np.zeros((max(m, 1), 3, 3))
+    library().el3dpartdirecttarg_(_dref(rlam), _dref(rmu), _iref(n), _p(src), _iref(ifs), _p(ssl), _iref(ifd),
+                                  _p(sdl), _p(sdv), _iref(ifptfrc), _p(pt), _iref(ifstrain), _p(st), _iref(m),
+                                  _p(tgt), _iref(ifptfrctarg if m else 0), _p(ptt),
+                                  _iref(ifstraintarg if m else 0), _p(stt))
+    return pt, st, ptt[:m], stt[:m]
+
+
+def elfmm3dparttarg_dev(iprec, rlam, rmu, source, sigma_sl, sigma_dl, sigma_dv, target, ptfrc, strain, ptfrctarg,
+                        straintarg, stream=None):
+    """Device-resident point-source apply on torch tensors (float64,
+    contiguous); any of sigma_sl / sigma_dl (with sigma_dv) / target /
+    outputs may be None. Returns ier."""
+    def ptr(t):
+        return C.c_void_p(0 if t is None else t.data_ptr())
+    n = source.shape[0]
+    m = 0 if target is None else target.shape[0]
+    s = C.c_void_p(0 if stream is None else stream.cuda_stream)
+    return library().elfmm_gfx950_parttarg_dev(
+        C.c_int(iprec), C.c_double(rlam), C.c_double(rmu), C.c_int(n), ptr(source),
+        C.c_int(sigma_sl is not None), ptr(sigma_sl), C.c_int(sigma_dl is not None), ptr(sigma_dl), ptr(sigma_dv),
+        C.c_int(ptfrc is not None), ptr(ptfrc), C.c_int(strain is not None), ptr(strain), C.c_int(m), ptr(target),
+        C.c_int(ptfrctarg is not None), ptr(ptfrctarg), C.c_int(straintarg is not None), ptr(straintarg), s)
 
 
 def elfmm3dtriatarg_dev(iprec, rlam, rmu, triangle, trinorm, source, sigma_sl, sigma_dl, target,

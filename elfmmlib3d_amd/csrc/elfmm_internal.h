@@ -17,6 +17,8 @@ void parallel_chunks(int nchunks, const std::function<void(int, int)> &fn);
 // lfmm3dtrilhesstarg (src/lfmm3dtrilhess.f:216-235): nbox = (120*2^k)/3.
 double iprec_eps(int iprec);
 int iprec_nbox(int iprec, int npts);
+// Leaf size of the point-source Laplace FMM (lfmm3dparthesstarg).
+int iprec_nbox_part(int iprec, int npts);
 // leaf size of the iter engine (src/lfmm3dtrilhess_iter_1.f:246-255): twice
 // the table above
 int iprec_nbox_iter(int iprec, int npts);

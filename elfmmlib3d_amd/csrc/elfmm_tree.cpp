@@ -48,6 +48,22 @@ int iprec_nbox(int iprec, int npts)
     }
 }
 
+int iprec_nbox_part(int iprec, int npts)
+{
+    // src/lfmm3dparthess.f:272-280
+    switch (iprec) {
+    case -2: return 40;
+    case -1: return 50;
+    case 0: return 80;
+    case 1: return 160;
+    case 2: return 400;
+    case 3: return 800;
+    case 4: return 1200;
+    case 5: return 1400;
+    default: return npts;
+    }
+}
+
 int iprec_nbox_iter(int iprec, int npts)
 {
     // src/lfmm3dtrilhess_iter_1.f:246-255 (bscale = 2)

@@ -183,6 +183,45 @@ void elh3dtriadirecttarg_(const double *rlam, const double *rmu, const double *t
                           const double *target, const int *ifptfrctarg, double *ptfrctarg,
                           const int *ifstraintarg, double *straintarg);
 
+/* Replaces elfmm3dparttarg (src/elfmm3dpart.f:104-109): the elastostatic
+ * FMM of point sources (point forces sigma_sl, force dipoles sigma_dl with
+ * orientations sigma_dv) as four Laplace FMMs on the particle FMM's octree
+ * (lfmm3dparthesstarg, leaf size src/lfmm3dparthess.f:272-280), the self
+ * pair excluded; evaluated at the sources (ptfrc, strain) and targets. Same
+ * error codes as elfmm3dtriatarg_. */
+void elfmm3dparttarg_(int *ier, const int *iprec, const double *rlam, const double *rmu, const int *nparts,
+                      const double *source, const int *ifsingle, const double *sigma_sl, const int *ifdouble,
+                      const double *sigma_dl, const double *sigma_dv, const int *ifptfrc, double *ptfrc,
+                      const int *ifstrain, double *strain, const int *ntargs, const double *target,
+                      const int *ifptfrctarg, double *ptfrctarg, const int *ifstraintarg, double *straintarg);
+
+/* Replaces elfmm3dpart (src/elfmm3dpart.f:52-55): elfmm3dparttarg_
+ * without targets. */
+void elfmm3dpart_(int *ier, const int *iprec, const double *rlam, const double *rmu, const int *nparts,
+                  const double *source, const int *ifsingle, const double *sigma_sl, const int *ifdouble,
+                  const double *sigma_dl, const double *sigma_dv, const int *ifptfrc, double *ptfrc,
+                  const int *ifstrain, double *strain);
+
+/* Replaces el3dpartdirecttarg (src/elfmm3dpart.f:506-511): the O(N^2)
+ * point-source sum without self pairs, on the GPU (the near-field kernel of
+ * the point FMM over one box holding every point). */
+void el3dpartdirecttarg_(const double *rlam, const double *rmu, const int *nsource, const double *source,
+                         const int *ifsingle, const double *sigma_sl, const int *ifdouble, const double *sigma_dl,
+                         const double *sigma_dv, const int *ifptfrc, double *ptfrc, const int *ifstrain,
+                         double *strain, const int *ntarget, const double *target, const int *ifptfrctarg,
+                         double *ptfrctarg, const int *ifstraintarg, double *straintarg);
+
+/* Device-resident elfmm3dparttarg: device pointers with the Fortran layouts
+ * above, enqueued on `stream` (a hipStream_t, NULL for the default stream),
+ * returns after the outputs are written; sigma_dv is read when ifdouble is
+ * 1. No counterpart in the reference: the same operation for callers whose
+ * data already lives in HBM. */
+int elfmm_gfx950_parttarg_dev(int iprec, double rlam, double rmu, int nparts, const double *d_source, int ifsingle,
+                              const double *d_sigma_sl, int ifdouble, const double *d_sigma_dl,
+                              const double *d_sigma_dv, int ifptfrc, double *d_ptfrc, int ifstrain, double *d_strain,
+                              int ntargs, const double *d_target, int ifptfrctarg, double *d_ptfrctarg,
+                              int ifstraintarg, double *d_straintarg, void *stream);
+
 /* Diagnostic: which parts of the half-space apply run (bit 0 image near
  * field, bit 1 anti-image far field, bit 2 Mindlin harmonic far field;
  * default 7), process-wide. Lets a test compare each part with the

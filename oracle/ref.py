@@ -410,3 +410,55 @@ def elh3dtriadirecttarg(rlam, rmu, triangle, trinorm, source, sigma_sl=None, sig
         _d(rlam), _d(rmu), _p(tri), _p(nrm), _i(n), _p(src), _i(ifs), _p(ssl), _i(ifd), _p(sdl), _i(ifptfrc),
         _p(pt), _i(ifstrain), _p(st), _i(m), _p(tgt), _i(ifptfrctarg), _p(ptt), _i(ifstraintarg), _p(stt))
     return pt, st, ptt[:m], stt[:m]
+
+
+def _part_inputs(source, sigma_sl, sigma_dl, sigma_dv, target):
+    n = source.shape[0]
+    src = _f64(source, (n, 3))
+    ifs = 0 if sigma_sl is None else 1
+    ifd = 0 if sigma_dl is None else 1
+    ssl = _f64(sigma_sl if ifs else np.zeros((n, 3)), (n, 3))
+    sdl = _f64(sigma_dl if ifd else np.zeros((n, 3)), (n, 3))
+    sdv = _f64(sigma_dv if ifd else np.zeros((n, 3)), (n, 3))
+    if target is None:
+        target = np.zeros((0, 3))
+    m = target.shape[0]
+    tgt = _f64(target if m else np.zeros((1, 3)), (max(m, 1), 3))
+    return n, m, src, ifs, ssl, ifd, sdl, sdv, tgt
+
+
+def elfmm3dparttarg(iprec, rlam, rmu, source, sigma_sl=None, sigma_dl=None, sigma_dv=None, target=None,
+                    ifptfrc=1, ifstrain=1, ifptfrctarg=1, ifstraintarg=1):
+    """Reference point-source FMM (src/elfmm3dpart.f:104-215). Returns
+    (ier, ptfrc[n,3], strain[n,3,3], ptfrctarg[m,3], straintarg[m,3,3])."""
+    n, m, src, ifs, ssl, ifd, sdl, sdv, tgt = _part_inputs(source, sigma_sl, sigma_dl, sigma_dv, target)
+    pt = np.zeros((n, 3))
+    st = np.zeros((n, 3, 3))
+    ptt = np.zeros((max(m, 1), 3))
+    stt = np.zeros((max(m, 1), 3, 3))
+    ier = C.c_int(0)
+    lib().elfmm3dparttarg_(
+        C.byref(ier), _i(iprec), _d(rlam), _d(rmu), _i(n), _p(src),
+        _i(ifs), _p(ssl), _i(ifd), _p(sdl), _p(sdv),
+        _i(ifptfrc), _p(pt), _i(ifstrain), _p(st),
+        _i(m), _p(tgt), _i(ifptfrctarg if m else 0), _p(ptt),
+        _i(ifstraintarg if m else 0), _p(stt))
+    return ier.value, pt, st, ptt[:m], stt[:m]
+
+
+def el3dpartdirecttarg(rlam, rmu, source, sigma_sl=None, sigma_dl=None, sigma_dv=None, target=None,
+                       ifptfrc=1, ifstrain=1, ifptfrctarg=1, ifstraintarg=1):
+    """Reference O(N^2) point-source sum without self pairs
+    (src/elfmm3dpart.f:506-712)."""
+    n, m, src, ifs, ssl, ifd, sdl, sdv, tgt = _part_inputs(source, sigma_sl, sigma_dl, sigma_dv, target)
+    pt = np.zeros((n, 3))
+    st = np.zeros((n, 3, 3))
+    ptt = np.zeros((max(m, 1), 3))
+    stt = np.zeros((max(m, 1), 3, 3))
+    lib().el3dpartdirecttarg_(
+        _d(rlam), _d(rmu), _i(n), _p(src),
+        _i(ifs), _p(ssl), _i(ifd), _p(sdl), _p(sdv),
+        _i(ifptfrc), _p(pt), _i(ifstrain), _p(st),
+        _i(m), _p(tgt), _i(ifptfrctarg if m else 0), _p(ptt),
+        _i(ifstraintarg if m else 0), _p(stt))
+    return pt, st, ptt[:m], stt[:m]

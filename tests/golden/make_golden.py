@@ -38,6 +38,13 @@ never need the reference itself. Cases (all FP64):
                 elh3dtriadirecttarg and the free-space iter FMM (direct
                 arrival) at 24 of the targets
   hs_fault_p1   the same geometry at iprec 1
+  part_s        point sources (src/elfmm3dpart.f): 2,400 points in the unit
+                cube plus a dense cluster of 1,200, 500 targets in [-0.5,1.5]^3,
+                lam=1.2 mu=0.8, iprec 2, single + double layer (random
+                dipole orientations), all outputs: elfmm3dparttarg
+  part_p1       the same points at iprec 1, double layer only, targets only
+  part_m1       the same points at iprec -1, single layer only, sources only
+  part_direct   el3dpartdirecttarg on 500 of the points + 80 targets
 """
 from __future__ import annotations
 
@@ -207,6 +214,28 @@ def c2_case(name, iprecs=(2, 3), nsample=3000, nproj=6):
     save(name, **out)
 
 
+def part_geometry(seed=41, nu=2400, ncl=1200, ntarg=500):
+    rng = np.random.default_rng(seed)
+    cl = rng.normal(0.0, 0.04, (ncl, 3)) + np.array([0.7, 0.3, 0.6])
+    src = np.concatenate([rng.uniform(0, 1, (nu, 3)), cl])
+    n = src.shape[0]
+    ssl = rng.uniform(-1, 1, (n, 3))
+    sdl = rng.uniform(-1, 1, (n, 3))
+    sdv = rng.normal(size=(n, 3))
+    sdv /= np.linalg.norm(sdv, axis=1, keepdims=True)
+    tg = rng.uniform(-0.5, 1.5, (ntarg, 3))
+    return src, ssl, sdl, sdv, tg
+
+
+def part_case(name, iprec, lam, mu, src, ssl, sdl, sdv, tg, flags=(1, 1, 1, 1)):
+    ier, pt, st, ptt, stt = ref.elfmm3dparttarg(iprec, lam, mu, src, ssl, sdl, sdv, tg, *flags)
+    assert ier == 0
+    z = np.zeros(0)
+    save(name, iprec=iprec, lam=lam, mu=mu, source=src, sigma_sl=ssl if ssl is not None else z,
+         sigma_dl=sdl if sdl is not None else z, sigma_dv=sdv if sdl is not None else z, target=tg,
+         flags=np.array(flags), ptfrc=pt, strain=st, ptfrctarg=ptt, straintarg=stt)
+
+
 def main(only=None):
     want = lambda name: only is None or name in only
     tri, nrm, src, ssl, sdl, tg = sphere_case(6, 300, 11)
@@ -241,6 +270,18 @@ def main(only=None):
     for name, iprec in (("iter_c1_p2", 2), ("iter_c1_p1", 1)):
         if want(name):
             iter_case(name, iprec)
+    src, ssl, sdl, sdv, tg = part_geometry()
+    if want("part_s"):
+        part_case("part_s", 2, 1.2, 0.8, src, ssl, sdl, sdv, tg)
+    if want("part_p1"):
+        part_case("part_p1", 1, 1.2, 0.8, src, None, sdl, sdv, tg, (0, 0, 1, 1))
+    if want("part_m1"):
+        part_case("part_m1", -1, 1.2, 0.8, src, ssl, None, None, np.zeros((0, 3)), (1, 1, 0, 0))
+    if want("part_direct"):
+        k = np.random.default_rng(43).choice(src.shape[0], 500, replace=False)
+        pt, st, ptt, stt = ref.el3dpartdirecttarg(1.2, 0.8, src[k], ssl[k], sdl[k], sdv[k], tg[:80])
+        save("part_direct", lam=1.2, mu=0.8, source=src[k], sigma_sl=ssl[k], sigma_dl=sdl[k], sigma_dv=sdv[k],
+             target=tg[:80], ptfrc=pt, strain=st, ptfrctarg=ptt, straintarg=stt)
     if only is not None:
         return
 

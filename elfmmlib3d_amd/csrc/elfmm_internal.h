@@ -159,8 +159,12 @@ void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool 
 // The two halves of build_plan, for a caller that starts the near field
 // before the far-field lists exist: build_plan_near fills the near-field
 // lists (nf_*, stats[6]) of P, build_plan_far everything else; neither
-// clears what the other fills (P starts as Plan() with zero stats).
+// clears what the other fills (P starts from plan_reset).
 void build_plan_near(const Tree &T, bool src_eval, bool direct_only, bool iter, Plan &P);
+// Empties P for a new plan (zero stats); the M2L pair arrays keep their
+// memory and length, which the next build_plan_far resizes and overwrites
+// (no fresh allocation or clearing of its largest arrays per apply).
+void plan_reset(Plan &P);
 void build_plan_far(const Tree &T, int n, const Operators *ops, bool src_eval, int nrhs, Plan &P, bool iter);
 
 // Task lists of the Mindlin harmonic far field (lfmm3d_list2_mindlin,

@@ -212,11 +212,23 @@ void build_plan_near(const Tree &T, bool src_eval, bool direct_only, bool iter, 
     P.stats[6] = npair_near;
 }
 
+void plan_reset(Plan &P)
+{
+    GemmLaunch keep;
+    keep.src.swap(P.m2l.src);
+    keep.dst.swap(P.m2l.dst);
+    keep.scale.swap(P.m2l.scale);
+    P = Plan();
+    P.m2l.src.swap(keep.src);
+    P.m2l.dst.swap(keep.dst);
+    P.m2l.scale.swap(keep.scale);
+    for (double &v : P.stats) v = 0;
+}
+
 void build_plan(const Tree &T, int n, const Operators *ops, bool src_eval, bool direct_only, int nrhs,
                 Plan &P, bool iter)
 {
-    P = Plan();
-    for (double &v : P.stats) v = 0;
+    plan_reset(P);
     build_plan_near(T, src_eval, direct_only, iter, P);
     build_plan_far(T, n, ops, src_eval, nrhs, P, iter);
 }
@@ -227,6 +239,11 @@ void build_plan_far(const Tree &T, int n, const Operators *ops, bool src_eval, i
     int p = ops ? ops->p : 0;
     int nc = ncoef(p);
     int nchunk = (nb + CHUNK - 1) / CHUNK;
+    if (!ops) {
+        P.m2l.src.clear();
+        P.m2l.dst.clear();
+        P.m2l.scale.clear();
+    }
     P.sides.resize(nb);
     for (int b = 0; b < nb; b++) P.sides[b] = T.box_side.empty() ? T.size / (double)(1 << T.level[b]) : T.box_side[b];
     auto npts = [&](int b) { return (src_eval ? T.src_count[b] : 0) + T.tgt_count[b]; };
@@ -274,8 +291,13 @@ void build_plan_far(const Tree &T, int n, const Operators *ops, bool src_eval, i
             ostart[o + 1] = run;
         }
         int npair = ostart[343];
-        std::vector<int> ps(npair), pd(npair);
-        std::vector<double> psc(npair);
+        // the pair arrays keep the previous plan's memory (plan_reset): the
+        // fill below writes every entry, so no allocation or clearing here
+        std::vector<int> &ps = P.m2l.src, &pd = P.m2l.dst;
+        std::vector<double> &psc = P.m2l.scale;
+        ps.resize(npair);
+        pd.resize(npair);
+        psc.resize(npair);
         parallel_chunks(nchunk, [&](int ch, int) {
             int *pos = &cnt[343 * (size_t)ch];
             for (int b = ch * CHUNK; b < std::min(nb, (ch + 1) * CHUNK); b++) {
@@ -335,9 +357,6 @@ void build_plan_far(const Tree &T, int n, const Operators *ops, bool src_eval, i
                     *t++ = M2LTile{ops->m2l_off[o], h, r0, ostart[o] + p0, std::min(ML_P, m - p0), ops->m2l_phi[o]};
         });
         xcd_order(P.m2l_tiles);
-        P.m2l.src.swap(ps);
-        P.m2l.dst.swap(pd);
-        P.m2l.scale.swap(psc);
         P.stats[3] = npair;
         P.stats[4] = dense;
         P.stats[5] = alg;

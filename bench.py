@@ -39,7 +39,7 @@ def workload(nsplit=64, ntarg=100000, seed=2024):
     return mesh.sphere_workload(nsplit, ntarg, seed)
 
 
-def cpu_baseline():
+def cpu_baseline(points_sample=None):
     """The reference's own FMM (oracle/_ref: flang -O1 -fopenmp, OpenMP over
     the host cores this process may use, OMP_STACKSIZE 1024M) on this host:
     C1 (icosphere 12x12/face, 2,880 triangles + 1,000 shell targets) as a
@@ -71,7 +71,17 @@ def cpu_baseline():
     ier, *_ = ref.elfmm3dtriatarg(2, 1.0, 1.0, *c2)
     c2_t = time.time() - t0
     pts = c2[0].shape[0] + c2[5].shape[0]
+    part = None
+    if points_sample is not None:
+        arrs, pg = points_sample
+        t0 = time.time()
+        ier, pr, *_ = ref.elfmm3dparttarg(2, 1.0, 1.0, *arrs, None, 1, 1, 0, 0)
+        tp = time.time() - t0
+        part = {"sample": "reference elfmm3dparttarg, 100,000 points uniform in the unit cube, SL + DL, "
+                          "iprec 2, one apply", "s_per_apply": tp, "value": 1e5 / tp / 1e6, "unit": "M points/s",
+                "gpu_vs_reference_rel": float(np.linalg.norm(pg - pr) / np.linalg.norm(pr))}
     return {"value": pts / c2_t / 1e6, "unit": "M tris+targets/s", "cores": threads, "kind": "reference",
+            "points": part,
             "sample": f"reference elfmm3dtriatarg (flang -O1 -fopenmp, OMP_STACKSIZE 1024M) on the full C2 "
                       f"workload, one apply after 5 C1 applies: {c2_t:.1f} s; {threads} threads of "
                       f"{os.cpu_count()} CPUs",
@@ -376,6 +386,54 @@ def c4_forest(args):
             "apply_ms": ms}
 
 
+def points_fmm(args):
+    """Point sources (elfmm3dparttarg): args.part_n points uniform in the unit
+    cube, SL + DL with random unit dipole orientations, displacement and
+    strain at the points, iprec 2 (400-point leaves), lam=mu=1; median of 4
+    applies after one warm-up. Also returns a 100,000-point sample of the
+    same kind with its GPU result, which the CPU-baseline leg runs through
+    the reference."""
+    import torch
+    import elfmmlib3d_amd as E
+    rng = np.random.default_rng(5)
+    dev = torch.device("cuda:0")
+
+    def draw(n):
+        src = rng.uniform(0, 1, (n, 3))
+        sdv = rng.normal(size=(n, 3))
+        sdv /= np.linalg.norm(sdv, axis=1, keepdims=True)
+        return src, rng.uniform(-1, 1, (n, 3)), rng.uniform(-1, 1, (n, 3)), sdv
+
+    def gpu_apply(arrs, reps):
+        T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        d = [T(a) for a in arrs]
+        n = arrs[0].shape[0]
+        out = [torch.empty((n, 3), dtype=torch.float64, device=dev),
+               torch.empty((n, 3, 3), dtype=torch.float64, device=dev)]
+        ts = []
+        for i in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ier = E.elfmm3dparttarg_dev(2, 1.0, 1.0, d[0], d[1], d[2], d[3], None, out[0], out[1], None, None)
+            torch.cuda.synchronize()
+            if ier != 0:
+                raise RuntimeError(f"elfmm3dparttarg ier={ier}")
+            if i:
+                ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), out[0].cpu().numpy()
+
+    n = args.part_n
+    t, _ = gpu_apply(draw(n), 4)
+    stats = E.last_stats()
+    res = {"workload": f"{n} points uniform in the unit cube, SL + DL, displacement and strain at the points, "
+                       "iprec 2, lam=mu=1", "value": n / t / 1e6, "unit": "M points/s", "apply_ms": 1e3 * t,
+           "near_pairs": stats["near_pairs"], "boxes": stats["nboxes"]}
+    sample = draw(100000)
+    ts, pg = gpu_apply(sample, 2)
+    res["sample_100k_gpu_ms"] = 1e3 * ts
+    return res, (sample, pg)
+
+
 def c5_served(args):
     """Config C5 on one GPU: a Poisson stream of requests at 0.5x, 1x and 2x
     the measured capacity; 60% half-space fault meshes (8 faults, targets =
@@ -519,6 +577,8 @@ def main():
     ap.add_argument("--c4-forest", action="store_true",
                     help="C4 meshes as the forest schedule only, three applies (for profiling)")
     ap.add_argument("--c5-max", type=int, default=500000, help="largest triangle count of a C5 request")
+    ap.add_argument("--part-n", type=int, default=1000000, help="points of the point-source FMM measurement")
+    ap.add_argument("--no-part", dest="part", action="store_false", help="skip the point-source FMM measurement")
     args = ap.parse_args()
     if args.kernel_pass:
         kernel_pass(args)
@@ -720,7 +780,8 @@ def main():
     c3 = c3_half(args) if args.c3 else None
     c4 = c4_batched(args) if args.c4 else None
     c5 = c5_served(args) if args.c5 else None
-    cpu = None if args.no_cpu else cpu_baseline()
+    pts, pts_sample = points_fmm(args) if args.part else (None, None)
+    cpu = None if args.no_cpu else cpu_baseline(pts_sample)
     line = {
         "metric": "FMM apply throughput (M tris+targets/s) @1e-6 err; p50/p99 served latency",
         "value": value, "unit": "M tris+targets/s", "n_gpus": 1, "steps": args.steps,
@@ -744,6 +805,7 @@ def main():
         "half_space_c3": c3,
         "batched_c4": c4,
         "served_c5": c5,
+        "points_fmm": pts,
         "traffic": ({"bytes_per_dispatch": traffic, "counter_scale": factors,
                      "method": f"rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, over a child run of two applies; "
                                f"KiB per dispatch scaled by the counters of a {CALIB_MIB} MiB device copy"}

@@ -579,6 +579,7 @@ def main():
     ap.add_argument("--c5-max", type=int, default=500000, help="largest triangle count of a C5 request")
     ap.add_argument("--part-n", type=int, default=1000000, help="points of the point-source FMM measurement")
     ap.add_argument("--no-part", dest="part", action="store_false", help="skip the point-source FMM measurement")
+    ap.add_argument("--part-only", action="store_true", help="run only the point-source FMM measurement")
     args = ap.parse_args()
     if args.kernel_pass:
         kernel_pass(args)
@@ -591,6 +592,9 @@ def main():
         return
     if args.c4_forest:
         print(json.dumps(c4_forest(args)), flush=True)
+        return
+    if args.part_only:
+        print(json.dumps(points_fmm(args)[0]), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "the library runs on one GPU per process; --gpus must be 1"}))

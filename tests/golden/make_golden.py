@@ -45,6 +45,8 @@ never need the reference itself. Cases (all FP64):
   part_p1       the same points at iprec 1, double layer only, targets only
   part_m1       the same points at iprec -1, single layer only, sources only
   part_direct   el3dpartdirecttarg on 500 of the points + 80 targets
+  part_tiny     elfmm3dparttarg with 2 points + 3 targets and with 1 point
+                (iprec 2, single + double layer)
 """
 from __future__ import annotations
 
@@ -277,6 +279,18 @@ def main(only=None):
         part_case("part_p1", 1, 1.2, 0.8, src, None, sdl, sdv, tg, (0, 0, 1, 1))
     if want("part_m1"):
         part_case("part_m1", -1, 1.2, 0.8, src, ssl, None, None, np.zeros((0, 3)), (1, 1, 0, 0))
+    if want("part_tiny"):
+        rng = np.random.default_rng(47)
+        src2, ssl2, sdl2 = rng.uniform(-1, 1, (2, 3)), rng.uniform(-1, 1, (2, 3)), rng.uniform(-1, 1, (2, 3))
+        sdv2 = rng.normal(size=(2, 3))
+        sdv2 /= np.linalg.norm(sdv2, axis=1, keepdims=True)
+        tg3 = rng.uniform(-2, 2, (3, 3))
+        ier2, pt2, st2, ptt2, stt2 = ref.elfmm3dparttarg(2, 1.1, 0.9, src2, ssl2, sdl2, sdv2, tg3)
+        ier1, pt1, st1, ptt1, stt1 = ref.elfmm3dparttarg(2, 1.1, 0.9, src2[:1], ssl2[:1], sdl2[:1], sdv2[:1], tg3)
+        assert ier2 == 0 and ier1 == 0
+        save("part_tiny", lam=1.1, mu=0.9, source=src2, sigma_sl=ssl2, sigma_dl=sdl2, sigma_dv=sdv2, target=tg3,
+             ptfrc2=pt2, strain2=st2, ptfrctarg2=ptt2, straintarg2=stt2,
+             ptfrc1=pt1, strain1=st1, ptfrctarg1=ptt1, straintarg1=stt1)
     if want("part_direct"):
         k = np.random.default_rng(43).choice(src.shape[0], 500, replace=False)
         pt, st, ptt, stt = ref.el3dpartdirecttarg(1.2, 0.8, src[k], ssl[k], sdl[k], sdv[k], tg[:80])

@@ -84,3 +84,21 @@ def test_point_device_entry_matches_host_entry(gpu_lib):
     assert ier2 == 0 and rel(pt, pt2) < 1e-14 and rel(st, st2) < 1e-14
     dpt, dst, _, _ = gpu_lib.el3dpartdirecttarg(*args)
     assert rel(pt, dpt) < 0.5e-6 and rel(st, dst) < 0.5e-6
+
+
+@pytest.mark.parametrize("npts", [1, 2])
+def test_point_fmm_tiny(gpu_lib, npts):
+    """One point (its self pair is the only pair: zero at the source) and two
+    points, with three targets: the reference's values (one-box trees,
+    near field only)."""
+    g = load_golden("part_tiny")
+    k = str(npts)
+    ier, pt, st, ptt, stt = gpu_lib.elfmm3dparttarg(2, float(g["lam"]), float(g["mu"]), g["source"][:npts],
+                                                    g["sigma_sl"][:npts], g["sigma_dl"][:npts],
+                                                    g["sigma_dv"][:npts], g["target"])
+    assert ier == 0
+    if npts == 1:
+        assert np.all(pt == 0) and np.all(st == 0)
+    else:
+        assert rel(pt, g["ptfrc" + k]) < 1e-13 and rel(st, g["strain" + k]) < 1e-13
+    assert rel(ptt, g["ptfrctarg" + k]) < 1e-13 and rel(stt, g["straintarg" + k]) < 1e-13

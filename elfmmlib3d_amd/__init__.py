@@ -18,6 +18,7 @@ from ._lib import (  # noqa: F401
     half_parts,
     elfmm3dtria,
     el3dpartdirecttarg,
+    elh3dpartdirecttarg,
     elfmm3dpart,
     elfmm3dparttarg,
     elfmm3dparttarg_dev,
@@ -36,7 +37,7 @@ from ._lib import (  # noqa: F401
 )
 from ._build import build  # noqa: F401
 
-__all__ = ["elfmm3dpart", "elfmm3dparttarg", "el3dpartdirecttarg", "elfmm3dparttarg_dev",
+__all__ = ["elfmm3dpart", "elfmm3dparttarg", "el3dpartdirecttarg", "elh3dpartdirecttarg", "elfmm3dparttarg_dev",
            "elfmm3dtria", "elfmm3dtriatarg", "el3dtriadirecttarg", "elfmm3dtriatarg_dev",
            "elfmm3dtriatargiter", "elhfmm3dtriatargiter", "elh3dtriadirecttarg", "half_parts",
            "iter_release",

@@ -263,6 +263,21 @@ def el3dpartdirecttarg(rlam, rmu, source, sigma_sl=None, sigma_dl=None, sigma_dv
     return pt, st, ptt[:m], stt[:m]
 
 
+def elh3dpartdirecttarg(rlam, rmu, source, sigma_sl=None, sigma_dl=None, sigma_dv=None, target=None,
+                        ifptfrc=1, ifstrain=1, ifptfrctarg=1, ifstraintarg=1):
+    """O(N^2) point-source sum with the Mindlin half-space Green's function,
+    points in z < 0 (src/elhfmm3dpart.f:961). Returns (ptfrc, strain,
+    ptfrctarg, straintarg)."""
+    n, src, ifs, ssl, ifd, sdl, sdv, m, tgt = _prep_part(source, sigma_sl, sigma_dl, sigma_dv, target)
+    pt, st = np.zeros((n, 3)), np.zeros((n, 3, 3))
+    ptt, stt = np.zeros((max(m, 1), 3)), np.zeros((max(m, 1), 3, 3))
+    library().elh3dpartdirecttarg_(_dref(rlam), _dref(rmu), _iref(n), _p(src), _iref(ifs), _p(ssl), _iref(ifd),
+                                   _p(sdl), _p(sdv), _iref(ifptfrc), _p(pt), _iref(ifstrain), _p(st), _iref(m),
+                                   _p(tgt), _iref(ifptfrctarg if m else 0), _p(ptt),
+                                   _iref(ifstraintarg if m else 0), _p(stt))
+    return pt, st, ptt[:m], stt[:m]
+
+
 def elfmm3dparttarg_dev(iprec, rlam, rmu, source, sigma_sl, sigma_dl, sigma_dv, target, ptfrc, strain, ptfrctarg,
                         straintarg, stream=None):
     """Device-resident point-source apply on torch tensors (float64,

@@ -31,6 +31,12 @@ hipError_t launch_hdirect(int npt, const double *pts, int n, const double *tri, 
                           const double *sdl, int ifsl, int ifdl, int ifpt, int ifst, double lam, double mu,
                           void *scratch, double *pt_out, double *st_out, hipStream_t st);
 hipError_t launch_mirror(int n, const double *in, double *out, hipStream_t st);
+// image part of the point-source half-space direct sum at the ne sources (if
+// ne > 0: ne == n) and nt targets, caller's order, added to the outputs
+hipError_t launch_pimage(int ne, int n, int nt, const double *src, const double *tgt, const double *ssl,
+                         const double *sdl, const double *sdv, int ifsl, int ifdl, double lam, double mu, int ifpt,
+                         double *pt, int ifst, double *st, int ifptt, double *ptt, int ifstt, double *stt,
+                         hipStream_t s);
 
 // ---- Mindlin harmonic far field (elfmm_mindlin.hip) ----
 struct MindArgs {

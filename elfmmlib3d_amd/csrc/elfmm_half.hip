@@ -418,4 +418,77 @@ hipError_t launch_mirror(int n, const double *in, double *out, hipStream_t st)
     return hipGetLastError();
 }
 
+// Image part of the point-source half-space direct sum (elh3dpartdirecttarg,
+// src/elhfmm3dpart.f:961-1178): at every evaluation point (the sources, when
+// ifpt or ifst, then the targets; caller's order) the image fields of all
+// sources, the point's own included as in the reference (image_sl with
+// sigma_sl, image_dl with sigma_dl and orientation sigma_dv), added to the
+// outputs. One thread per point; the sources go through LDS in tiles.
+constexpr int PI_THREADS = 128, PI_TILE = 128, PI_REC = 12;
+__global__ __launch_bounds__(PI_THREADS) void k_pimage(int ne, int n, int nt, const double *__restrict__ src,
+                                                      const double *__restrict__ tgt, const double *__restrict__ ssl,
+                                                      const double *__restrict__ sdl, const double *__restrict__ sdv,
+                                                      int ifsl, int ifdl, double lam, double mu, int ifpt,
+                                                      double *__restrict__ pt, int ifst, double *__restrict__ st,
+                                                      int ifptt, double *__restrict__ ptt, int ifstt,
+                                                      double *__restrict__ stt)
+{
+    __shared__ double S[PI_TILE * PI_REC];
+    int ip = blockIdx.x * PI_THREADS + threadIdx.x;
+    bool has = ip < ne + nt;
+    bool is_src = ip < ne;
+    const double *y = has ? (is_src ? src + 3 * (size_t)ip : tgt + 3 * (size_t)(ip - ne)) : nullptr;
+    double Y[3] = {0.0, 0.0, -1.0};
+    if (has) Y[0] = y[0], Y[1] = y[1], Y[2] = y[2];
+    double u[3] = {0, 0, 0}, g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int t0 = 0; t0 < n; t0 += PI_TILE) {
+        int m = min(PI_TILE, n - t0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < m * PI_REC; e += PI_THREADS) {
+            int sj = e / PI_REC, c = e - sj * PI_REC;
+            size_t j = (size_t)(t0 + sj);
+            double v = c < 3 ? src[3 * j + c]
+                             : c < 6 ? (ifsl ? ssl[3 * j + c - 3] : 0.0)
+                                     : c < 9 ? (ifdl ? sdl[3 * j + c - 6] : 0.0) : (ifdl ? sdv[3 * j + c - 9] : 0.0);
+            S[e] = v;
+        }
+        __syncthreads();
+        if (!has) continue;
+        for (int sj = 0; sj < m; sj++) {
+            const double *r = S + sj * PI_REC;
+            double f[12];
+            if (ifsl) {
+                image_sl(lam, mu, r, r + 3, Y, 1, 1, f);
+                for (int d = 0; d < 3; d++) u[d] += f[d];
+                for (int d = 0; d < 9; d++) g[d] += f[3 + d];
+            }
+            if (ifdl) {
+                image_dl(lam, mu, r, r + 6, r + 9, Y, 1, 1, f);
+                for (int d = 0; d < 3; d++) u[d] += f[d];
+                for (int d = 0; d < 9; d++) g[d] += f[3 + d];
+            }
+        }
+    }
+    if (!has) return;
+    double *po = is_src ? (ifpt ? pt + 3 * (size_t)ip : nullptr) : (ifptt ? ptt + 3 * (size_t)(ip - ne) : nullptr);
+    double *so = is_src ? (ifst ? st + 9 * (size_t)ip : nullptr) : (ifstt ? stt + 9 * (size_t)(ip - ne) : nullptr);
+    if (po)
+        for (int d = 0; d < 3; d++) po[d] += u[d];
+    if (so)
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) so[3 * b + a] += g[3 * a + b];
+}
+
+hipError_t launch_pimage(int ne, int n, int nt, const double *src, const double *tgt, const double *ssl,
+                         const double *sdl, const double *sdv, int ifsl, int ifdl, double lam, double mu, int ifpt,
+                         double *pt, int ifst, double *st, int ifptt, double *ptt, int ifstt, double *stt,
+                         hipStream_t s)
+{
+    int npt = ne + nt;
+    if (npt <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pimage, dim3((npt + PI_THREADS - 1) / PI_THREADS), dim3(PI_THREADS), 0, s, ne, n, nt, src,
+                       tgt, ssl, sdl, sdv, ifsl, ifdl, lam, mu, ifpt, pt, ifst, st, ifptt, ptt, ifstt, stt);
+    return hipGetLastError();
+}
+
 }  // namespace elfmm

@@ -95,6 +95,20 @@ module elfmm_gfx950
        real(c_double) :: target(3, *), ptfrctarg(3, *), straintarg(3, 3, *)
      end subroutine el3dpartdirecttarg
 
+     ! src/elhfmm3dpart.f:961-966 (points in z < 0)
+     subroutine elh3dpartdirecttarg(rlam, rmu, nsource, source, &
+          ifsingle, sigma_sl, ifdouble, sigma_dl, sigma_dv, ifptfrc, ptfrc, ifstrain, strain, &
+          ntarget, target, ifptfrctarg, ptfrctarg, ifstraintarg, straintarg) &
+          bind(C, name="elh3dpartdirecttarg_")
+       import :: c_int, c_double
+       integer(c_int) :: nsource, ifsingle, ifdouble, ifptfrc, ifstrain
+       integer(c_int) :: ntarget, ifptfrctarg, ifstraintarg
+       real(c_double) :: rlam, rmu
+       real(c_double) :: source(3, *), sigma_sl(3, *), sigma_dl(3, *), sigma_dv(3, *)
+       real(c_double) :: ptfrc(3, *), strain(3, 3, *)
+       real(c_double) :: target(3, *), ptfrctarg(3, *), straintarg(3, 3, *)
+     end subroutine elh3dpartdirecttarg
+
      ! src/elfmm3dtria_iter_1.f:160-166 (wsave carries a handle to the
      ! device-resident near-field interactions; lused = 4 for icomp 1-5)
      subroutine elfmm3dtriatargiter(ier, iprec, rlam, rmu, triangle, trinorm, nparts, source, &

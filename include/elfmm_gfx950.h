@@ -211,6 +211,16 @@ void el3dpartdirecttarg_(const double *rlam, const double *rmu, const int *nsour
                          double *strain, const int *ntarget, const double *target, const int *ifptfrctarg,
                          double *ptfrctarg, const int *ifstraintarg, double *straintarg);
 
+/* Replaces elh3dpartdirecttarg (src/elhfmm3dpart.f:961-966): the O(N^2)
+ * point-source sum with the Mindlin half-space Green's function (points in
+ * z < 0): the free-space sum without self pairs plus the image part of
+ * every source, its own point included, on the GPU. */
+void elh3dpartdirecttarg_(const double *rlam, const double *rmu, const int *nsource, const double *source,
+                          const int *ifsingle, const double *sigma_sl, const int *ifdouble, const double *sigma_dl,
+                          const double *sigma_dv, const int *ifptfrc, double *ptfrc, const int *ifstrain,
+                          double *strain, const int *ntarget, const double *target, const int *ifptfrctarg,
+                          double *ptfrctarg, const int *ifstraintarg, double *straintarg);
+
 /* Device-resident elfmm3dparttarg: device pointers with the Fortran layouts
  * above, enqueued on `stream` (a hipStream_t, NULL for the default stream),
  * returns after the outputs are written; sigma_dv is read when ifdouble is

@@ -462,3 +462,22 @@ def el3dpartdirecttarg(rlam, rmu, source, sigma_sl=None, sigma_dl=None, sigma_dv
         _i(m), _p(tgt), _i(ifptfrctarg if m else 0), _p(ptt),
         _i(ifstraintarg if m else 0), _p(stt))
     return pt, st, ptt[:m], stt[:m]
+
+
+def elh3dpartdirecttarg(rlam, rmu, source, sigma_sl=None, sigma_dl=None, sigma_dv=None, target=None,
+                        ifptfrc=1, ifstrain=1, ifptfrctarg=1, ifstraintarg=1):
+    """Reference half-space point-source direct sum (src/elhfmm3dpart.f:961-1178):
+    the Mindlin Green's function, the free-space part of the self pair
+    excluded. Points in z < 0."""
+    n, m, src, ifs, ssl, ifd, sdl, sdv, tgt = _part_inputs(source, sigma_sl, sigma_dl, sigma_dv, target)
+    pt = np.zeros((n, 3))
+    st = np.zeros((n, 3, 3))
+    ptt = np.zeros((max(m, 1), 3))
+    stt = np.zeros((max(m, 1), 3, 3))
+    libh().elh3dpartdirecttarg_(
+        _d(rlam), _d(rmu), _i(n), _p(src),
+        _i(ifs), _p(ssl), _i(ifd), _p(sdl), _p(sdv),
+        _i(ifptfrc), _p(pt), _i(ifstrain), _p(st),
+        _i(m), _p(tgt), _i(ifptfrctarg if m else 0), _p(ptt),
+        _i(ifstraintarg if m else 0), _p(stt))
+    return pt, st, ptt[:m], stt[:m]

@@ -47,6 +47,8 @@ never need the reference itself. Cases (all FP64):
   part_direct   el3dpartdirecttarg on 500 of the points + 80 targets
   part_tiny     elfmm3dparttarg with 2 points + 3 targets and with 1 point
                 (iprec 2, single + double layer)
+  hpart_direct  elh3dpartdirecttarg (half space) on 400 points in z < 0
+                (depths 0.02-2) + 60 targets in z < 0, single + double layer
 """
 from __future__ import annotations
 
@@ -291,6 +293,16 @@ def main(only=None):
         save("part_tiny", lam=1.1, mu=0.9, source=src2, sigma_sl=ssl2, sigma_dl=sdl2, sigma_dv=sdv2, target=tg3,
              ptfrc2=pt2, strain2=st2, ptfrctarg2=ptt2, straintarg2=stt2,
              ptfrc1=pt1, strain1=st1, ptfrctarg1=ptt1, straintarg1=stt1)
+    if want("hpart_direct"):
+        rng = np.random.default_rng(53)
+        src = np.column_stack([rng.uniform(-1, 1, (400, 2)), -rng.uniform(0.02, 2.0, 400)])
+        tgh = np.column_stack([rng.uniform(-1.5, 1.5, (60, 2)), -rng.uniform(0.01, 2.5, 60)])
+        ssl, sdl = rng.uniform(-1, 1, (400, 3)), rng.uniform(-1, 1, (400, 3))
+        sdv = rng.normal(size=(400, 3))
+        sdv /= np.linalg.norm(sdv, axis=1, keepdims=True)
+        pt, st, ptt, stt = ref.elh3dpartdirecttarg(1.3, 0.7, src, ssl, sdl, sdv, tgh)
+        save("hpart_direct", lam=1.3, mu=0.7, source=src, sigma_sl=ssl, sigma_dl=sdl, sigma_dv=sdv, target=tgh,
+             ptfrc=pt, strain=st, ptfrctarg=ptt, straintarg=stt)
     if want("part_direct"):
         k = np.random.default_rng(43).choice(src.shape[0], 500, replace=False)
         pt, st, ptt, stt = ref.el3dpartdirecttarg(1.2, 0.8, src[k], ssl[k], sdl[k], sdv[k], tg[:80])

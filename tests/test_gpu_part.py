@@ -102,3 +102,19 @@ def test_point_fmm_tiny(gpu_lib, npts):
     else:
         assert rel(pt, g["ptfrc" + k]) < 1e-13 and rel(st, g["strain" + k]) < 1e-13
     assert rel(ptt, g["ptfrctarg" + k]) < 1e-13 and rel(stt, g["straintarg" + k]) < 1e-13
+
+
+def test_half_space_point_direct_matches_reference(gpu_lib):
+    """elh3dpartdirecttarg: 400 points and 60 targets in z < 0, single and
+    double layer; the free-space sum without self pairs plus the image of
+    every source (its own point included)."""
+    g = load_golden("hpart_direct")
+    args = (float(g["lam"]), float(g["mu"]), g["source"], g["sigma_sl"], g["sigma_dl"], g["sigma_dv"], g["target"])
+    pt, st, ptt, stt = gpu_lib.elh3dpartdirecttarg(*args)
+    assert rel(pt, g["ptfrc"]) < 1e-12
+    assert rel(st, g["strain"]) < 1e-12
+    assert rel(ptt, g["ptfrctarg"]) < 1e-12
+    assert rel(stt, g["straintarg"]) < 1e-12
+    # the image part alone is the difference from the free-space sum
+    fpt, _, fptt, _ = gpu_lib.el3dpartdirecttarg(*args)
+    assert rel(pt - fpt, g["ptfrc"] - fpt) < 1e-11 and rel(ptt - fptt, g["ptfrctarg"] - fptt) < 1e-11

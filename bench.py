@@ -9,9 +9,15 @@ requests with --workers applies in flight (library calls on separate host
 threads and streams, so host planning and kernels of different requests
 overlap); value = (triangles + targets) x K / wall time, latency = per
 request from submission to result. The same K applies back to back on one
-stream are reported as single_request. Prints one JSON line.
+stream are reported as single_request. Beside the C2 line: the C3 half-space
+apply and its 64-slip-vector call (half_space_c3), the C4 batch of 256
+meshes (batched_c4), the C5 served stream (served_c5), the point-source FMM
+on 1 M points (points_fmm), the iter replay's HBM roofline, the per-kernel
+HBM traffic (rocprofv3 child runs) and the reference's CPU time on the host
+cores (cpu_baseline). Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workers 3] [--iprec 2] [--no-cpu]
+                  [--no-c3] [--no-c4] [--no-c5] [--no-part] [--no-traffic]
 """
 import argparse
 import json

@@ -78,6 +78,9 @@ struct Operators {
     // (real parts, then imaginary parts; see build_operators).
     int m2l_pt[343];
     int64_t m2l_off[343];
+    // 1 when offset o uses the stored operator of its mirror image in z:
+    // R(i, j, -k) = S R(i, j, k) S with S = diag((-1)^(n+m))
+    int m2l_refl[343];
     double m2l_phi[343];
     std::vector<double> m2l;
 };
@@ -108,6 +111,7 @@ struct M2LTile {
     int pair0;       // first pair of this tile
     int npair;       // <= ML_P
     double phi;      // azimuth of the translation
+    int refl = 0;    // 1: the stored operator is that of the z-mirrored offset (signs (-1)^(n+m) on both sides)
 };
 constexpr int ML_M = 64, ML_P = 32;
 

@@ -198,7 +198,11 @@ void build_operators(int p, const int *itable, Operators &ops)
     // offset: the real-part matrix, then the imaginary-part matrix.
     // The rotated operator depends on the offset only through rho^2 =
     // i^2 + j^2, k and pt: offsets that share them (up to eight, mirror
-    // images in x, y and the diagonal) share one stored copy.
+    // images in x, y and the diagonal) share one stored copy. The mirror
+    // image in z shares it too: the irregular harmonics of the translation
+    // satisfy I_n^0(rho, 0, k) = (-1)^n I_n^0(rho, 0, -k) term by term, so
+    // R(i, j, -k) = S R(i, j, k) S with S = diag((-1)^(n+m)) on the packed
+    // coefficients (m2l_refl; the kernel applies S with the azimuth phase).
     size_t tot = 0;
     std::vector<int> owner(343, -1);
     for (int o = 0; o < 343; o++) {
@@ -208,12 +212,15 @@ void build_operators(int p, const int *itable, Operators &ops)
         ops.m2l_pt[o] = far ? pt : -1;
         ops.m2l_phi[o] = far ? std::atan2(-(double)j, -(double)i) : 0.0;
         ops.m2l_off[o] = (int64_t)tot;
+        ops.m2l_refl[o] = 0;
         if (!far) continue;
         for (int q = 0; q < o; q++) {
             int iq = q / 49 - 3, jq = (q / 7) % 7 - 3, kq = q % 7 - 3;
-            if (ops.m2l_pt[q] == pt && kq == k && iq * iq + jq * jq == i * i + j * j) {
+            if (owner[q] < 0 && ops.m2l_pt[q] == pt && std::abs(kq) == std::abs(k) &&
+                iq * iq + jq * jq == i * i + j * j) {
                 owner[o] = q;
                 ops.m2l_off[o] = ops.m2l_off[q];
+                ops.m2l_refl[o] = kq != k ? 1 : 0;
                 break;
             }
         }

@@ -19,7 +19,7 @@ namespace elfmm {
 namespace {
 
 void add_pairs(GemmLaunch &g, int64_t a_off, int h, double phi, const int *src, const int *dst, const double *scale,
-               int np)
+               int np, int refl = 0)
 {
     int base = (int)g.src.size();
     g.src.insert(g.src.end(), src, src + np);
@@ -27,7 +27,7 @@ void add_pairs(GemmLaunch &g, int64_t a_off, int h, double phi, const int *src, 
     g.scale.insert(g.scale.end(), scale, scale + np);
     for (int p0 = 0; p0 < np; p0 += ML_P)
         for (int r0 = 0; r0 < h; r0 += ML_M)
-            g.tiles.push_back({a_off, h, r0, base + p0, std::min(ML_P, np - p0), phi});
+            g.tiles.push_back({a_off, h, r0, base + p0, std::min(ML_P, np - p0), phi, refl});
 }
 
 // Launch order of translation tiles: workgroup b of a launch runs on XCD
@@ -354,7 +354,8 @@ void build_plan_far(const Tree &T, int n, const Operators *ops, bool src_eval, i
             M2LTile *t = &P.m2l_tiles[tstart[o]];
             for (int p0 = 0; p0 < m; p0 += ML_P)
                 for (int r0 = 0; r0 < h; r0 += ML_M)
-                    *t++ = M2LTile{ops->m2l_off[o], h, r0, ostart[o] + p0, std::min(ML_P, m - p0), ops->m2l_phi[o]};
+                    *t++ = M2LTile{ops->m2l_off[o], h, r0, ostart[o] + p0, std::min(ML_P, m - p0), ops->m2l_phi[o],
+                                   ops->m2l_refl[o]};
         });
         xcd_order(P.m2l_tiles);
         P.stats[3] = npair;
@@ -559,7 +560,7 @@ void build_mindlin_plan(const Tree &T, const Operators &ops, MindlinPlan &MP)
             sc[i - k] = (double)(1 << T.level[d[i - k]]) / T.size;
         }
         int h = ncoef(ops.m2l_pt[o]) / 2;
-        add_pairs(g, ops.m2l_off[o], h, ops.m2l_phi[o], s.data(), d.data(), sc.data(), (int)(e - k));
+        add_pairs(g, ops.m2l_off[o], h, ops.m2l_phi[o], s.data(), d.data(), sc.data(), (int)(e - k), ops.m2l_refl[o]);
         k = e;
     }
     MP.m2l_tiles = g.tiles;
